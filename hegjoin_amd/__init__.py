@@ -1,0 +1,296 @@
+"""hegjoin_amd -- MI355X-native HEGJoin epsilon-grid self-join (algorithm 0).
+
+Python host mirror of the C ABI in include/hegjoin.h (libhegjoin.so, built
+in-tree by hegjoin_amd/build.py).  The names follow the reference's host
+entry points:
+
+    GridIndex(database, epsilon, ...)  <- generateNDGridDimensions +
+                                          gridIndexingGPU (main.cu:449-520, GPU.cu:74-398)
+    GridIndex.sort_by_workload()       <- sortByWorkLoad (SortByWorkload.cu:19-247)
+    GridIndex.selfjoin()               <- distanceTableNDGridBatches (GPU.cu:787-1702)
+    NeighborTable.csr()/counts()       <- constructNeighborTableKeyValueWithPtrs
+                                          (GPU.cu:1731-1776), as CSR by point id
+
+The library is required: importing this package raises ImportError when
+libhegjoin.so has not been built, and every call raises HegjError on a
+non-zero hegj_status.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libhegjoin.so")
+MAX_INDEXED_DIMS = 6
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is not built; run `python -m hegjoin_amd.build`")
+_lib = C.CDLL(LIB_PATH)
+
+GRID_DTYPE = np.dtype([("indexmin", np.int32), ("indexmax", np.int32)])
+LOOKUP_DTYPE = np.dtype([("idx", np.uint32), ("pad", np.uint32), ("gridLinearID", np.uint64)])
+SCHED_DTYPE = np.dtype([("nbPoints", np.int32), ("cellId", np.int32)])
+
+STATUS = {0: "ok", 1: "invalid argument", 2: "unsupported", 3: "no device",
+          4: "out of device memory", 5: "HIP error", 6: "result capacity exceeded",
+          7: "I/O error", 8: "busy"}
+
+
+class HegjError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"hegj status {status} ({STATUS.get(status, '?')}): {msg}")
+        self.status = status
+
+
+class _IndexOpts(C.Structure):
+    _fields_ = [("dim", C.c_uint32), ("indexed_dims", C.c_uint32), ("dtype", C.c_uint32),
+                ("emulate_cell0", C.c_uint32), ("device", C.c_int32)]
+
+
+class _IndexInfo(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("dim", C.c_uint32), ("indexed_dims", C.c_uint32),
+                ("dtype", C.c_uint32), ("nNonEmptyCells", C.c_uint32), ("n_rows", C.c_uint32),
+                ("n_cell0_queries", C.c_uint32), ("emulate_cell0", C.c_uint32),
+                ("totalCells", C.c_uint64), ("epsilon", C.c_double), ("threshold", C.c_double),
+                ("minArr", C.c_double * MAX_INDEXED_DIMS), ("maxArr", C.c_double * MAX_INDEXED_DIMS),
+                ("nCells", C.c_uint32 * MAX_INDEXED_DIMS), ("build_ms", C.c_double)]
+
+
+class _JoinOpts(C.Structure):
+    _fields_ = [("result_capacity", C.c_uint64), ("unit_order", C.c_uint32),
+                ("queries_per_unit", C.c_uint32), ("block_size", C.c_uint32),
+                ("reserved", C.c_uint32 * 3)]
+
+
+class _ResultInfo(C.Structure):
+    _fields_ = [("total_neighbors", C.c_uint64), ("pair_hash", C.c_uint64),
+                ("total_candidates", C.c_uint64), ("evals_executed", C.c_uint64),
+                ("band_rechecks", C.c_uint64), ("n_queries", C.c_uint32),
+                ("n_batches", C.c_uint32), ("n_units", C.c_uint32), ("n_deferred", C.c_uint32),
+                ("estimate_ms", C.c_double), ("join_ms", C.c_double), ("workload_ms", C.c_double),
+                ("total_ms", C.c_double), ("join_kernel_avg_ms", C.c_double),
+                ("join_launches", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+def _struct_dict(s):
+    out = {}
+    for name, _ in s._fields_:
+        v = getattr(s, name)
+        if name in ("reserved",):
+            continue
+        if hasattr(v, "__len__"):
+            v = list(v)
+        out[name] = v
+    return out
+
+
+_vp = C.c_void_p
+_lib.hegj_status_string.restype = C.c_char_p
+_lib.hegj_last_error.restype = C.c_char_p
+_lib.hegj_version.restype = C.c_char_p
+_lib.hegj_dataset_generate_expo.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, _vp]
+_lib.hegj_dataset_load.argtypes = [C.c_char_p, C.c_uint32, C.POINTER(C.POINTER(C.c_double)),
+                                   C.POINTER(C.c_uint32)]
+_lib.hegj_dataset_free.argtypes = [_vp]
+_lib.hegj_index_opts_default.argtypes = [C.POINTER(_IndexOpts), C.c_uint32]
+_lib.hegj_index_build.argtypes = [_vp, C.c_uint32, C.c_double, C.POINTER(_IndexOpts), C.POINTER(_vp)]
+_lib.hegj_index_build_device.argtypes = _lib.hegj_index_build.argtypes
+_lib.hegj_index_free.argtypes = [_vp]
+_lib.hegj_index_get_info.argtypes = [_vp, C.POINTER(_IndexInfo)]
+_lib.hegj_index_copy_grid.argtypes = [_vp, _vp, _vp, _vp]
+_lib.hegj_index_copy_cell_ids.argtypes = [_vp, _vp]
+_lib.hegj_sort_by_workload.argtypes = [_vp, _vp, _vp, C.POINTER(C.c_uint64)]
+_lib.hegj_join_opts_default.argtypes = [C.POINTER(_JoinOpts)]
+_lib.hegj_selfjoin.argtypes = [_vp, C.POINTER(_JoinOpts), C.POINTER(_vp)]
+_lib.hegj_result_free.argtypes = [_vp]
+_lib.hegj_result_get_info.argtypes = [_vp, C.POINTER(_ResultInfo)]
+_lib.hegj_result_copy_counts.argtypes = [_vp, _vp]
+_lib.hegj_result_copy_csr.argtypes = [_vp, _vp, _vp, C.c_int]
+
+
+def _check(status):
+    if status != 0:
+        raise HegjError(status, _lib.hegj_last_error().decode())
+
+
+def version() -> str:
+    return _lib.hegj_version().decode()
+
+
+def device_count() -> int:
+    return _lib.hegj_device_count()
+
+
+def library():
+    """The loaded libhegjoin.so (ctypes.CDLL)."""
+    return _lib
+
+
+def generate_expo(n, dim, seed=0, lam=40.0):
+    """The reference's dataset generator (datasets/dataset_gen_fixed_len_expo_dist.cpp):
+    mt19937(seed) + exponential(lam), values redrawn outside [0, 1]."""
+    out = np.empty((n, dim), np.float64)
+    _check(_lib.hegj_dataset_generate_expo(seed, n, dim, lam, out.ctypes.data))
+    return out
+
+
+def load_dataset(path, dim):
+    """importNDDataset-compatible text loader (*.bin: raw row-major doubles)."""
+    p = C.POINTER(C.c_double)()
+    n = C.c_uint32()
+    _check(_lib.hegj_dataset_load(str(path).encode(), dim, C.byref(p), C.byref(n)))
+    try:
+        return np.ctypeslib.as_array(p, shape=(n.value * dim,)).copy().reshape(n.value, dim) \
+            if n.value else np.zeros((0, dim))
+    finally:
+        _lib.hegj_dataset_free(p)
+
+
+class GridIndex:
+    """Device-resident grid index over a dataset (reference: gridIndexingGPU).
+
+    database: (n, dim) float64 host array, or a device pointer (int) with
+    n and dim given.  indexed_dims = NUMINDEXEDDIM (default min(dim, 6)).
+    emulate_cell0 reproduces the reference's cell-0 wrap (default True).
+    """
+
+    def __init__(self, database, epsilon, indexed_dims=None, emulate_cell0=True, device=0,
+                 n=None, dim=None):
+        self._h = _vp()
+        if isinstance(database, int):
+            if n is None or dim is None:
+                raise ValueError("n and dim are required with a device pointer")
+            ptr, build = database, _lib.hegj_index_build_device
+            self._keep = None
+        else:
+            arr = np.ascontiguousarray(database, dtype=np.float64)
+            if arr.ndim != 2:
+                raise ValueError("database must be (n, dim)")
+            n, dim = arr.shape
+            ptr, build = arr.ctypes.data, _lib.hegj_index_build
+            self._keep = arr
+        o = _IndexOpts()
+        _lib.hegj_index_opts_default(C.byref(o), dim)
+        if indexed_dims is not None:
+            o.indexed_dims = indexed_dims
+        o.emulate_cell0 = 1 if emulate_cell0 else 0
+        o.device = device
+        _check(build(ptr, n, float(epsilon), C.byref(o), C.byref(self._h)))
+        self._keep = None
+        self.n, self.dim = int(n), int(dim)
+
+    def close(self):
+        if self._h:
+            _lib.hegj_index_free(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def info(self) -> dict:
+        ii = _IndexInfo()
+        _check(_lib.hegj_index_get_info(self._h, C.byref(ii)))
+        d = _struct_dict(ii)
+        k = d["indexed_dims"]
+        for key in ("minArr", "maxArr", "nCells"):
+            d[key] = d[key][:k]
+        return d
+
+    def copy_grid(self):
+        """(index[C] grid, gridCellLookupArr[C], indexLookupArr[n]) in the
+        reference's struct layouts (structs.h:83-98)."""
+        c = self.info["nNonEmptyCells"]
+        grid = np.zeros(c, GRID_DTYPE)
+        lookup = np.zeros(c, LOOKUP_DTYPE)
+        ila = np.zeros(self.n, np.uint32)
+        _check(_lib.hegj_index_copy_grid(self._h, grid.ctypes.data, lookup.ctypes.data,
+                                         ila.ctypes.data))
+        return grid, lookup, ila
+
+    def cell_ids(self):
+        ids = np.zeros(self.n, np.uint64)
+        _check(_lib.hegj_index_copy_cell_ids(self._h, ids.ctypes.data))
+        return ids
+
+    def sort_by_workload(self):
+        """(sortedCells[C] {nbPoints, cellId}, originPointIndex[n], totalCandidates)."""
+        c = self.info["nNonEmptyCells"]
+        cells = np.zeros(c, SCHED_DTYPE)
+        opi = np.zeros(self.n, np.uint32)
+        tot = C.c_uint64()
+        _check(_lib.hegj_sort_by_workload(self._h, cells.ctypes.data, opi.ctypes.data,
+                                          C.byref(tot)))
+        return cells, opi, tot.value
+
+    def selfjoin(self, result_capacity=0, unit_order="spatial", queries_per_unit=0):
+        o = _JoinOpts()
+        _lib.hegj_join_opts_default(C.byref(o))
+        o.result_capacity = result_capacity
+        o.unit_order = {"spatial": 0, "workload": 1}[unit_order]
+        o.queries_per_unit = queries_per_unit
+        r = _vp()
+        _check(_lib.hegj_selfjoin(self._h, C.byref(o), C.byref(r)))
+        return NeighborTable(r, self)
+
+
+class NeighborTable:
+    """Result of a self-join: per-query neighbour rows on the device."""
+
+    def __init__(self, handle, index):
+        self._h = handle
+        self._index = index  # the result refers to the index's device arrays
+        self.n = index.n
+
+    def close(self):
+        if self._h:
+            _lib.hegj_result_free(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def info(self) -> dict:
+        ri = _ResultInfo()
+        _check(_lib.hegj_result_get_info(self._h, C.byref(ri)))
+        return _struct_dict(ri)
+
+    def counts(self):
+        c = np.zeros(self.n, np.uint32)
+        _check(_lib.hegj_result_copy_counts(self._h, c.ctypes.data))
+        return c
+
+    def csr(self, sort_rows=True):
+        """(offsets[n+1] uint64, neighbors[total] uint32), rows by point id."""
+        total = self.info["total_neighbors"]
+        off = np.zeros(self.n + 1, np.uint64)
+        nb = np.zeros(max(total, 1), np.uint32)
+        _check(_lib.hegj_result_copy_csr(self._h, off.ctypes.data, nb.ctypes.data,
+                                         1 if sort_rows else 0))
+        return off, nb[:total]
+
+
+def selfjoin(database, epsilon, indexed_dims=None, emulate_cell0=True, device=0, **kw):
+    """One-shot: build the index, join, return (offsets, neighbors, info)."""
+    with GridIndex(database, epsilon, indexed_dims, emulate_cell0, device) as ix:
+        t = ix.selfjoin(**kw)
+        off, nb = t.csr()
+        info = t.info
+        t.close()
+        return off, nb, info

@@ -1,0 +1,566 @@
+// hegj_api.cpp -- the extern "C" boundary of libhegjoin (include/hegjoin.h).
+//
+// Host orchestration of the device pipeline that replaces the reference's
+// gridIndexingGPU / sortByWorkLoad / distanceTableNDGridBatches.  Host code
+// here only sizes buffers, launches kernels on the handle's stream and reads
+// back scalars; every per-point and per-pair step runs on the device.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "hegj_internal.h"
+
+namespace hegj {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+hegj_status fail(hegj_status s, const std::string &msg) {
+    g_last_error = msg;
+    return s;
+}
+
+hipError_t Scratch::alloc(void **p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipSuccess) ptrs.push_back(*p);
+    return e;
+}
+
+Scratch::~Scratch() {
+    for (void *p : ptrs) (void)hipFree(p);
+}
+
+// S7: largest s with RN(sqrt(s)) <= eps.  sqrt is monotone, so
+// sqrt(acc) <= eps  <=>  acc <= T, and the device needs no sqrt.
+static double sqrt_threshold(double eps) {
+    double s = eps * eps;
+    const double inf = std::numeric_limits<double>::infinity();
+    while (s > 0 && std::sqrt(s) > eps) s = std::nextafter(s, 0.0);
+    while (std::sqrt(std::nextafter(s, inf)) <= eps) s = std::nextafter(s, inf);
+    return s;
+}
+
+struct EventTimer {
+    hipEvent_t a = nullptr, b = nullptr;
+    hipStream_t s;
+    explicit EventTimer(hipStream_t st) : s(st) {
+        (void)hipEventCreate(&a);
+        (void)hipEventCreate(&b);
+        (void)hipEventRecord(a, s);
+    }
+    float stop() {
+        float ms = 0;
+        (void)hipEventRecord(b, s);
+        (void)hipEventSynchronize(b);
+        (void)hipEventElapsedTime(&ms, a, b);
+        return ms;
+    }
+    ~EventTimer() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
+}  // namespace hegj
+
+using namespace hegj;
+
+struct hegj_index {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    GridGeom g{};
+    double maxArr[kMaxK]{};
+    uint64_t totalCells = 0;
+    DevIndex ix;
+    uint32_t n_dead = 0;
+    double build_ms = 0;
+    // workload sort (lazily computed)
+    uint32_t *d_work = nullptr;   // [C] candidates per cell
+    uint32_t *d_order = nullptr;  // [C] cells by workload
+    uint64_t total_candidates = 0;
+    double workload_ms = 0;
+    bool have_workload = false;
+};
+
+struct hegj_result {
+    hegj_index *idx = nullptr;
+    std::vector<uint32_t *> batches;
+    std::vector<uint64_t> bases;  // global offset of each batch
+    uint64_t *d_row_off = nullptr;
+    uint32_t *d_row_cnt = nullptr;
+    hegj_result_info info{};
+    ~hegj_result() {
+        for (uint32_t *b : batches) (void)hipFree(b);
+        if (d_row_off) (void)hipFree(d_row_off);
+        if (d_row_cnt) (void)hipFree(d_row_cnt);
+    }
+};
+
+namespace {
+
+hegj_status set_device(int dev) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        return fail(HEGJ_ERR_NO_DEVICE, "no HIP device visible");
+    if (dev < 0 || dev >= count) return fail(HEGJ_ERR_INVALID_ARG, "device ordinal out of range");
+    HIP_TRY(hipSetDevice(dev));
+    return HEGJ_OK;
+}
+
+hegj_status check_opts(const hegj_index_opts *o, uint32_t n, double eps) {
+    if (!o) return fail(HEGJ_ERR_INVALID_ARG, "opts is NULL");
+    if (n == 0) return fail(HEGJ_ERR_INVALID_ARG, "empty dataset");
+    if (o->dim == 0) return fail(HEGJ_ERR_INVALID_ARG, "dim must be >= 1");
+    if (o->indexed_dims == 0 || o->indexed_dims > HEGJ_MAX_INDEXED_DIMS || o->indexed_dims > o->dim)
+        return fail(HEGJ_ERR_INVALID_ARG, "indexed_dims must be in [1, min(dim, 6)]");
+    if (o->dtype != HEGJ_DTYPE_F64) return fail(HEGJ_ERR_UNSUPPORTED, "only DTYPE=double is built");
+    if (!(eps > 0.0) || eps > 1.0) return fail(HEGJ_ERR_INVALID_ARG, "epsilon must be in ]0, 1]");
+    return HEGJ_OK;
+}
+
+hegj_status build_common(hegj_index *h, uint32_t n, double eps, const hegj_index_opts *o) {
+    h->g.n = n;
+    h->g.dim = o->dim;
+    h->g.k = o->indexed_dims;
+    h->g.eps = eps;
+    h->g.thresh = sqrt_threshold(eps);
+    h->g.emulate_cell0 = o->emulate_cell0 ? 1u : 0u;
+    EventTimer t(h->stream);
+    HEGJ_TRY(build_index(&h->g, h->maxArr, &h->totalCells, &h->ix, &h->n_dead, h->stream));
+    h->build_ms = t.stop();
+    return HEGJ_OK;
+}
+
+template <typename F>
+hegj_status new_index(const hegj_index_opts *o, hegj_index **out, F fill) {
+    std::unique_ptr<hegj_index> h(new hegj_index());
+    h->device = o->device;
+    HEGJ_TRY(set_device(o->device));
+    HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    hegj_status s = fill(h.get());
+    if (s != HEGJ_OK) {
+        std::string msg = g_last_error;
+        hegj_index_free(h.release());
+        g_last_error = msg;
+        return s;
+    }
+    *out = h.release();
+    return HEGJ_OK;
+}
+
+hegj_status ensure_workload(hegj_index *h) {
+    if (h->have_workload) return HEGJ_OK;
+    const uint32_t C = h->ix.C;
+    HIP_TRY(hipMalloc(&h->d_work, sizeof(uint32_t) * (C + 1)));
+    HIP_TRY(hipMalloc(&h->d_order, sizeof(uint32_t) * (C + 1)));
+    EventTimer t(h->stream);
+    HEGJ_TRY(launch_workload(h->g, h->ix, h->d_work, h->stream));
+    HEGJ_TRY(workload_order(h->ix, h->d_work, h->d_order, &h->total_candidates, h->stream));
+    h->workload_ms = t.stop();
+    h->have_workload = true;
+    return HEGJ_OK;
+}
+
+// Default queries per work unit by dimensionality (tuned on MI355X).
+uint32_t default_qb(uint32_t dim) {
+    if (dim <= 2) return 16;
+    if (dim <= 8) return 8;
+    return 8;
+}
+
+struct Counters {
+    uint32_t queue_head;
+    uint32_t n_deferred;
+    unsigned long long cursor;
+    unsigned long long hash;
+    unsigned long long evals;
+    unsigned long long pairs;
+};
+
+}  // namespace
+
+extern "C" {
+
+const char *hegj_status_string(hegj_status s) {
+    switch (s) {
+        case HEGJ_OK: return "ok";
+        case HEGJ_ERR_INVALID_ARG: return "invalid argument";
+        case HEGJ_ERR_UNSUPPORTED: return "unsupported";
+        case HEGJ_ERR_NO_DEVICE: return "no device";
+        case HEGJ_ERR_OUT_OF_MEMORY: return "out of device memory";
+        case HEGJ_ERR_HIP: return "HIP error";
+        case HEGJ_ERR_CAPACITY: return "result capacity exceeded";
+        case HEGJ_ERR_IO: return "I/O error";
+        case HEGJ_ERR_BUSY: return "busy";
+    }
+    return "unknown";
+}
+
+const char *hegj_last_error(void) { return g_last_error.c_str(); }
+
+const char *hegj_version(void) { return "hegjoin-mi355x 0.1 (gfx950)"; }
+
+int hegj_device_count(void) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+void hegj_index_opts_default(hegj_index_opts *o, uint32_t dim) {
+    o->dim = dim;
+    o->indexed_dims = dim < (uint32_t)HEGJ_MAX_INDEXED_DIMS ? dim : HEGJ_MAX_INDEXED_DIMS;
+    o->dtype = HEGJ_DTYPE_F64;
+    o->emulate_cell0 = 1;
+    o->device = 0;
+}
+
+hegj_status hegj_index_build(const double *database, uint32_t n, double epsilon,
+                             const hegj_index_opts *opts, hegj_index **out) {
+    if (!database || !out) return fail(HEGJ_ERR_INVALID_ARG, "NULL argument");
+    HEGJ_TRY(check_opts(opts, n, epsilon));
+    return new_index(opts, out, [&](hegj_index *h) -> hegj_status {
+        size_t bytes = sizeof(double) * (size_t)n * opts->dim;
+        HIP_TRY(hipMalloc(&h->ix.pts, bytes));
+        HIP_TRY(hipMemcpyAsync(h->ix.pts, database, bytes, hipMemcpyHostToDevice, h->stream));
+        return build_common(h, n, epsilon, opts);
+    });
+}
+
+hegj_status hegj_index_build_device(const double *dev_database, uint32_t n, double epsilon,
+                                    const hegj_index_opts *opts, hegj_index **out) {
+    if (!dev_database || !out) return fail(HEGJ_ERR_INVALID_ARG, "NULL argument");
+    HEGJ_TRY(check_opts(opts, n, epsilon));
+    return new_index(opts, out, [&](hegj_index *h) -> hegj_status {
+        size_t bytes = sizeof(double) * (size_t)n * opts->dim;
+        HIP_TRY(hipMalloc(&h->ix.pts, bytes));
+        HIP_TRY(hipMemcpyAsync(h->ix.pts, dev_database, bytes, hipMemcpyDeviceToDevice, h->stream));
+        return build_common(h, n, epsilon, opts);
+    });
+}
+
+hegj_status hegj_index_free(hegj_index *h) {
+    if (!h) return HEGJ_OK;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    free_index(&h->ix);
+    if (h->d_work) (void)hipFree(h->d_work);
+    if (h->d_order) (void)hipFree(h->d_order);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return HEGJ_OK;
+}
+
+hegj_status hegj_index_get_info(const hegj_index *h, hegj_index_info *info) {
+    if (!h || !info) return fail(HEGJ_ERR_INVALID_ARG, "NULL argument");
+    std::memset(info, 0, sizeof(*info));
+    info->n = h->g.n;
+    info->dim = h->g.dim;
+    info->indexed_dims = h->g.k;
+    info->dtype = HEGJ_DTYPE_F64;
+    info->nNonEmptyCells = h->ix.C;
+    info->n_rows = h->ix.R;
+    info->n_cell0_queries = h->n_dead;
+    info->emulate_cell0 = h->g.emulate_cell0;
+    info->totalCells = h->totalCells;
+    info->epsilon = h->g.eps;
+    info->threshold = h->g.thresh;
+    for (uint32_t j = 0; j < h->g.k; j++) {
+        info->minArr[j] = h->g.minArr[j];
+        info->maxArr[j] = h->maxArr[j];
+        info->nCells[j] = h->g.nCells[j];
+    }
+    info->build_ms = h->build_ms;
+    return HEGJ_OK;
+}
+
+hegj_status hegj_index_copy_grid(const hegj_index *h, hegj_grid *index,
+                                 hegj_grid_cell_lookup *lookup, uint32_t *ila) {
+    if (!h) return fail(HEGJ_ERR_INVALID_ARG, "NULL index");
+    HEGJ_TRY(set_device(h->device));
+    const uint32_t C = h->ix.C, n = h->g.n;
+    if (index || lookup) {
+        std::vector<uint32_t> start(C + 1);
+        std::vector<uint64_t> keys(C);
+        HIP_TRY(hipMemcpyAsync(start.data(), h->ix.cell_start, sizeof(uint32_t) * (C + 1),
+                               hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipMemcpyAsync(keys.data(), h->ix.cell_key, sizeof(uint64_t) * C,
+                               hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        for (uint32_t c = 0; c < C; c++) {
+            if (index) index[c] = hegj_grid{(int32_t)start[c], (int32_t)start[c + 1] - 1};
+            if (lookup) lookup[c] = hegj_grid_cell_lookup{c, 0u, keys[c]};
+        }
+    }
+    if (ila) {
+        HIP_TRY(hipMemcpyAsync(ila, h->ix.ila, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+    }
+    return HEGJ_OK;
+}
+
+hegj_status hegj_index_copy_cell_ids(const hegj_index *h, uint64_t *ids) {
+    if (!h || !ids) return fail(HEGJ_ERR_INVALID_ARG, "NULL argument");
+    HEGJ_TRY(set_device(h->device));
+    HIP_TRY(hipMemcpyAsync(ids, h->ix.point_key, sizeof(uint64_t) * h->g.n, hipMemcpyDeviceToHost,
+                           h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return HEGJ_OK;
+}
+
+hegj_status hegj_sort_by_workload(hegj_index *h, hegj_scheduling_cell *sorted, uint32_t *opi,
+                                  uint64_t *total_candidates) {
+    if (!h) return fail(HEGJ_ERR_INVALID_ARG, "NULL index");
+    HEGJ_TRY(set_device(h->device));
+    HEGJ_TRY(ensure_workload(h));
+    const uint32_t C = h->ix.C, n = h->g.n;
+    if (total_candidates) *total_candidates = h->total_candidates;
+    if (sorted) {
+        std::vector<uint32_t> order(C), work(C);
+        HIP_TRY(hipMemcpyAsync(order.data(), h->d_order, sizeof(uint32_t) * C, hipMemcpyDeviceToHost,
+                               h->stream));
+        HIP_TRY(hipMemcpyAsync(work.data(), h->d_work, sizeof(uint32_t) * C, hipMemcpyDeviceToHost,
+                               h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        for (uint32_t i = 0; i < C; i++)
+            sorted[i] = hegj_scheduling_cell{(int32_t)work[order[i]], (int32_t)order[i]};
+    }
+    if (opi) {
+        Scratch sc;
+        uint32_t *d_opi;
+        HIP_TRY(sc.get(&d_opi, n));
+        HEGJ_TRY(origin_point_index(h->ix, h->d_order, d_opi, h->stream));
+        HIP_TRY(hipMemcpyAsync(opi, d_opi, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+    }
+    return HEGJ_OK;
+}
+
+void hegj_join_opts_default(hegj_join_opts *o) {
+    std::memset(o, 0, sizeof(*o));
+    o->unit_order = HEGJ_ORDER_SPATIAL;
+}
+
+hegj_status hegj_selfjoin(hegj_index *h, const hegj_join_opts *opts_in, hegj_result **out) {
+    if (!h || !out) return fail(HEGJ_ERR_INVALID_ARG, "NULL argument");
+    auto t_start = std::chrono::steady_clock::now();
+    HEGJ_TRY(set_device(h->device));
+    hegj_join_opts opts;
+    if (opts_in)
+        opts = *opts_in;
+    else
+        hegj_join_opts_default(&opts);
+    const uint32_t qb = opts.queries_per_unit ? opts.queries_per_unit : default_qb(h->g.dim);
+    if (qb != 4 && qb != 8 && qb != 16)
+        return fail(HEGJ_ERR_INVALID_ARG, "queries_per_unit must be 4, 8 or 16");
+    hipStream_t s = h->stream;
+    const uint32_t n = h->g.n;
+
+    std::unique_ptr<hegj_result> r(new hegj_result());
+    r->idx = h;
+    HEGJ_TRY(ensure_workload(h));
+    r->info.total_candidates = h->total_candidates;
+    r->info.workload_ms = h->workload_ms;
+    r->info.n_queries = n - h->n_dead;
+
+    Scratch sc;
+    uint2 *units = nullptr;
+    uint32_t U = 0;
+    const uint32_t *order = opts.unit_order == HEGJ_ORDER_WORKLOAD ? h->d_order : nullptr;
+    HEGJ_TRY(build_units(h->g, h->ix, qb, order, &units, &U, s));
+    sc.ptrs.push_back(units);
+    r->info.n_units = U;
+
+    HIP_TRY(hipMalloc(&r->d_row_off, sizeof(uint64_t) * n));
+    HIP_TRY(hipMalloc(&r->d_row_cnt, sizeof(uint32_t) * n));
+    HIP_TRY(hipMemsetAsync(r->d_row_off, 0, sizeof(uint64_t) * n, s));
+    HIP_TRY(hipMemsetAsync(r->d_row_cnt, 0, sizeof(uint32_t) * n, s));
+
+    Counters *d_ctr;
+    uint32_t *deferred;
+    uint2 *units_next;
+    HIP_TRY(sc.get(&d_ctr, 1));
+    HIP_TRY(sc.get(&deferred, U + 1));
+    HIP_TRY(sc.get(&units_next, U + 1));
+
+    uint32_t max_blocks = 0;
+    HEGJ_TRY(join_occupancy(h->g.dim, qb, h->device, &max_blocks));
+
+    JoinArgs a{};
+    a.g = h->g;
+    a.ix = h->ix;
+    a.queue_head = &d_ctr->queue_head;
+    a.n_deferred = &d_ctr->n_deferred;
+    a.cursor = &d_ctr->cursor;
+    a.hash = &d_ctr->hash;
+    a.evals = &d_ctr->evals;
+    a.pairs = &d_ctr->pairs;
+    a.row_off = r->d_row_off;
+    a.row_cnt = r->d_row_cnt;
+    a.deferred = deferred;
+
+    auto grab_for = [&](uint32_t nu) {
+        // about 32 grabs per wave keeps the queue atomic off the critical path
+        uint64_t waves = (uint64_t)max_blocks * 4;
+        uint64_t g = nu / (waves * 32);
+        return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, 64));
+    };
+    auto blocks_for = [&](uint32_t nu, uint32_t grab) {
+        uint64_t need = ((uint64_t)nu + grab * 4 - 1) / (grab * 4);
+        return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(need, max_blocks));
+    };
+
+    // -- batch estimate: count-only pass over a strided unit sample ------------
+    uint64_t capacity = opts.result_capacity;
+    if (capacity == 0 && U > 0) {
+        EventTimer t(s);
+        uint32_t stride = std::max<uint32_t>(1, U / 65536);
+        uint32_t ns = (U + stride - 1) / stride;
+        uint2 *sample;
+        HIP_TRY(sc.get(&sample, ns));
+        HEGJ_TRY(stride_units(units, ns, stride, sample, s));
+        HIP_TRY(hipMemsetAsync(d_ctr, 0, sizeof(Counters), s));
+        JoinArgs e = a;
+        e.units = sample;
+        e.n_units = ns;
+        e.units_per_grab = grab_for(ns);
+        e.count_only = 1;
+        HEGJ_TRY(launch_join(e, qb, blocks_for(ns, e.units_per_grab), s));
+        Counters c;
+        HIP_TRY(hipMemcpyAsync(&c, d_ctr, sizeof(c), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        double est = (double)c.pairs * ((double)U / (double)ns);
+        capacity = (uint64_t)(est * 1.10) + (1u << 20);
+        size_t free_b = 0, total_b = 0;
+        HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+        uint64_t cap_mem = (uint64_t)(free_b * 0.6) / sizeof(uint32_t);
+        capacity = std::min(capacity, cap_mem);
+        r->info.estimate_ms = t.stop();
+    }
+    if (capacity == 0) capacity = 1;
+
+    // -- batches -----------------------------------------------------------------
+    HIP_TRY(hipMemsetAsync(d_ctr, 0, sizeof(Counters), s));
+    const uint2 *cur = units;
+    uint32_t cur_n = U;
+    uint64_t base = 0;
+    double join_ms = 0;
+    uint32_t launches = 0;
+    Counters c{};
+    while (cur_n > 0) {
+        uint32_t *buf = nullptr;
+        HIP_TRY(hipMalloc(&buf, sizeof(uint32_t) * capacity));
+        r->batches.push_back(buf);
+        r->bases.push_back(base);
+        // reset per-batch counters (hash / evals / pairs accumulate)
+        HIP_TRY(hipMemsetAsync(d_ctr, 0, offsetof(Counters, hash), s));
+        a.units = cur;
+        a.n_units = cur_n;
+        a.units_per_grab = grab_for(cur_n);
+        a.out = buf;
+        a.out_base = base;
+        a.capacity = capacity;
+        a.count_only = 0;
+        {
+            EventTimer t(s);
+            HEGJ_TRY(launch_join(a, qb, blocks_for(cur_n, a.units_per_grab), s));
+            join_ms += t.stop();
+            launches++;
+        }
+        HIP_TRY(hipMemcpyAsync(&c, d_ctr, sizeof(c), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (r->batches.size() >= (size_t)kMaxBatches && c.n_deferred)
+            return fail(HEGJ_ERR_CAPACITY, "result needs more than 64 batches");
+        base += capacity;
+        if (c.n_deferred == cur_n) {
+            // nothing fit: one unit alone exceeds the batch; grow it
+            (void)hipFree(r->batches.back());
+            r->batches.pop_back();
+            r->bases.pop_back();
+            base -= capacity;
+            capacity *= 2;
+            continue;
+        }
+        r->info.n_deferred += c.n_deferred;
+        if (c.n_deferred) {
+            // deferred[] indexes the current list; gather into the other one
+            uint2 *nxt = (cur == units) ? units_next : units;
+            HEGJ_TRY(gather_units(cur, deferred, c.n_deferred, nxt, s));
+            cur = nxt;
+        }
+        cur_n = c.n_deferred;
+    }
+    r->info.total_neighbors = c.pairs;
+    r->info.pair_hash = c.hash;
+    r->info.evals_executed = c.evals;
+    r->info.n_batches = (uint32_t)r->batches.size();
+    r->info.join_ms = join_ms;
+    r->info.join_launches = launches;
+    r->info.join_kernel_avg_ms = launches ? join_ms / launches : 0.0;
+    r->info.total_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    *out = r.release();
+    return HEGJ_OK;
+}
+
+hegj_status hegj_result_free(hegj_result *r) {
+    if (!r) return HEGJ_OK;
+    (void)hipSetDevice(r->idx->device);
+    delete r;
+    return HEGJ_OK;
+}
+
+hegj_status hegj_result_get_info(const hegj_result *r, hegj_result_info *info) {
+    if (!r || !info) return fail(HEGJ_ERR_INVALID_ARG, "NULL argument");
+    *info = r->info;
+    return HEGJ_OK;
+}
+
+hegj_status hegj_result_copy_counts(const hegj_result *r, uint32_t *counts) {
+    if (!r || !counts) return fail(HEGJ_ERR_INVALID_ARG, "NULL argument");
+    HEGJ_TRY(set_device(r->idx->device));
+    hipStream_t s = r->idx->stream;
+    HIP_TRY(hipMemcpyAsync(counts, r->d_row_cnt, sizeof(uint32_t) * r->idx->g.n,
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return HEGJ_OK;
+}
+
+hegj_status hegj_result_copy_csr(const hegj_result *r, uint64_t *offsets, uint32_t *nbrs,
+                                 int sort_rows) {
+    if (!r || !offsets) return fail(HEGJ_ERR_INVALID_ARG, "NULL argument");
+    HEGJ_TRY(set_device(r->idx->device));
+    hipStream_t s = r->idx->stream;
+    const uint32_t n = r->idx->g.n;
+    Scratch sc;
+    uint64_t *d_off;
+    HIP_TRY(sc.get(&d_off, (size_t)n + 1));
+    HEGJ_TRY(csr_offsets(r->d_row_cnt, n, d_off, s));
+    HIP_TRY(hipMemcpyAsync(offsets, d_off, sizeof(uint64_t) * (n + 1), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const uint64_t total = offsets[n];
+    if (!nbrs || total == 0) return HEGJ_OK;
+    uint32_t *d_nb;
+    HIP_TRY(sc.get(&d_nb, total));
+    HEGJ_TRY(gather_rows(r->batches, r->bases, r->d_row_off, r->d_row_cnt, d_off, n, d_nb, s));
+    HIP_TRY(hipMemcpyAsync(nbrs, d_nb, sizeof(uint32_t) * total, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (sort_rows) {
+        unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nt; t++)
+            th.emplace_back([&, t]() {
+                for (uint64_t q = t; q < n; q += nt) std::sort(nbrs + offsets[q], nbrs + offsets[q + 1]);
+            });
+        for (auto &x : th) x.join();
+    }
+    return HEGJ_OK;
+}
+
+}  // extern "C"

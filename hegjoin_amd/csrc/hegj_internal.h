@@ -1,0 +1,268 @@
+// hegj_internal.h -- shared declarations of libhegjoin (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/hegjoin.h"
+
+namespace hegj {
+
+// ---------------------------------------------------------------------------
+// Error plumbing: every HIP call goes through HIP_TRY, which records a message
+// for hegj_last_error() and returns HEGJ_ERR_HIP from the calling function.
+// ---------------------------------------------------------------------------
+void set_error(const std::string &msg);
+hegj_status fail(hegj_status s, const std::string &msg);
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t _e = (expr);                                                         \
+        if (_e != hipSuccess) {                                                         \
+            return ::hegj::fail(_e == hipErrorOutOfMemory ? HEGJ_ERR_OUT_OF_MEMORY      \
+                                                          : HEGJ_ERR_HIP,               \
+                                std::string(#expr) + ": " + hipGetErrorString(_e));     \
+        }                                                                               \
+    } while (0)
+
+#define HEGJ_TRY(expr)                        \
+    do {                                      \
+        hegj_status _s = (expr);              \
+        if (_s != HEGJ_OK) return _s;         \
+    } while (0)
+
+constexpr int kWave = 64;          // CDNA wavefront width
+constexpr int kMaxK = HEGJ_MAX_INDEXED_DIMS;
+constexpr int kMaxRows = 243;      // 3^(kMaxK-1) stencil rows
+constexpr int kMaxBatches = 64;    // result batches per join
+
+// Grid geometry passed by value to kernels.
+struct GridGeom {
+    double minArr[kMaxK];
+    uint32_t nCells[kMaxK];
+    double eps;
+    double thresh;      // largest s with sqrt(s) <= eps
+    uint32_t k;
+    uint32_t dim;
+    uint32_t n;
+    uint32_t emulate_cell0;
+};
+
+// Device-resident grid index.  Naming follows the reference: a "cell" is a
+// non-empty grid cell (gridCellLookupArr entry), a "row" groups the cells that
+// share indexed coordinates 1..k-1 (they differ only in coordinate 0 and are
+// contiguous in linear-id order).
+struct DevIndex {
+    double *pts = nullptr;        // [n*dim] AoS input copy (dev_database)
+    double *soa = nullptr;        // [dim][n] coordinates in cell order
+    uint32_t *ila = nullptr;      // [n] indexLookupArr: cell-order position -> point id
+    uint64_t *point_key = nullptr;// [n] linear cell id per point (original order)
+    uint64_t *cell_key = nullptr; // [C] gridLinearID of each non-empty cell
+    uint32_t *cell_start = nullptr;// [C+1] first cell-order position of each cell
+    uint32_t *row_start = nullptr;// [R+1] first cell of each row
+    uint64_t *row_key = nullptr;  // [R] row key = cell_key / nCells[0]
+    uint64_t *hkeys = nullptr;    // row hash table keys (empty = ~0)
+    uint32_t *hvals = nullptr;    // row hash table values (row index)
+    uint32_t hmask = 0;
+    uint32_t C = 0, R = 0;
+};
+
+// Scratch allocator for temporaries of one call (freed by the destructor).
+struct Scratch {
+    std::vector<void *> ptrs;
+    hipError_t alloc(void **p, size_t bytes);
+    template <typename T>
+    hipError_t get(T **p, size_t count) {
+        return alloc(reinterpret_cast<void **>(p), count * sizeof(T) + 16);
+    }
+    ~Scratch();
+};
+
+// ---- primitives (hegj_prims.hip) -----------------------------------------
+// Exclusive scan; writes the total to *d_total when non-null.
+hegj_status scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *d_total,
+                     hipStream_t s);
+hegj_status scan_u64(const uint64_t *in, uint64_t *out, uint32_t n, uint64_t *d_total,
+                     hipStream_t s);
+// Stable LSD radix sort of (key, value) pairs on the low `bits` key bits.
+// Sorted data ends in keys/vals (the alt buffers are clobbered).
+hegj_status radix_sort_u64(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt,
+                           uint32_t *vals_alt, uint32_t n, int bits, hipStream_t s);
+hegj_status radix_sort_u32(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt,
+                           uint32_t *vals_alt, uint32_t n, int bits, hipStream_t s);
+
+// Arguments of the join kernel (hegj_join.hip).
+struct JoinArgs {
+    GridGeom g;
+    DevIndex ix;
+    const uint2 *units;           // (cell, first query offset in cell)
+    uint32_t n_units;
+    uint32_t units_per_grab;
+    uint32_t *queue_head;
+    uint32_t *out;                // this batch's neighbour buffer
+    uint64_t out_base;            // global offset of out[0]
+    uint64_t capacity;            // entries in out
+    unsigned long long *cursor;   // entries reserved in out
+    uint64_t *row_off;            // [n] global offset per query id
+    uint32_t *row_cnt;            // [n]
+    uint32_t *deferred;           // units that did not fit
+    uint32_t *n_deferred;
+    unsigned long long *hash;     // multiset hash of written pairs
+    unsigned long long *evals;    // distance evaluations performed
+    unsigned long long *pairs;    // pairs found (count-only) or written
+    uint32_t count_only;
+};
+
+// ---- index (hegj_index.hip) ----------------------------------------------
+hegj_status build_index(GridGeom *g, double *maxArr, uint64_t *totalCells, DevIndex *ix,
+                        uint32_t *n_dead, hipStream_t s);
+void free_index(DevIndex *ix);
+
+// ---- join (hegj_join.hip) ------------------------------------------------
+hegj_status launch_workload(const GridGeom &g, const DevIndex &ix, uint32_t *work, hipStream_t s);
+hegj_status build_units(const GridGeom &g, const DevIndex &ix, uint32_t qb, const uint32_t *order,
+                        uint2 **units, uint32_t *n_units, hipStream_t s);
+hegj_status launch_join(const JoinArgs &a, uint32_t qb, uint32_t blocks, hipStream_t s);
+// Workgroups of the persistent join kernel that fit on the device at once.
+hegj_status join_occupancy(uint32_t dim, uint32_t qb, int device, uint32_t *blocks);
+
+// ---- utilities (hegj_util.hip) -------------------------------------------
+hegj_status workload_order(const DevIndex &ix, const uint32_t *work, uint32_t *order,
+                           uint64_t *total_candidates, hipStream_t s);
+hegj_status origin_point_index(const DevIndex &ix, const uint32_t *order, uint32_t *opi,
+                               hipStream_t s);
+hegj_status stride_units(const uint2 *in, uint32_t n_out, uint32_t stride, uint2 *out,
+                         hipStream_t s);
+hegj_status gather_units(const uint2 *in, const uint32_t *idx, uint32_t n, uint2 *out,
+                         hipStream_t s);
+hegj_status csr_offsets(const uint32_t *row_cnt, uint32_t n, uint64_t *csr_off, hipStream_t s);
+hegj_status gather_rows(const std::vector<uint32_t *> &batches, const std::vector<uint64_t> &bases,
+                        const uint64_t *row_off, const uint32_t *row_cnt, const uint64_t *csr_off,
+                        uint32_t n, uint32_t *out, hipStream_t s);
+
+// ---- device helpers --------------------------------------------------------
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+    uint64_t z = x + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ uint32_t hash_slot(uint64_t key) {
+    return (uint32_t)(mix64(key) >> 16);
+}
+
+// Lanes below the calling lane.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ uint32_t wave_bcast_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+__device__ __forceinline__ uint64_t wave_bcast_u64(uint64_t v) {
+    uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Row hash lookup: returns the row index or ~0u.
+__device__ __forceinline__ uint32_t row_lookup(const uint64_t *__restrict__ hkeys,
+                                               const uint32_t *__restrict__ hvals,
+                                               uint32_t hmask, uint64_t key) {
+    uint32_t s = hash_slot(key) & hmask;
+    while (true) {
+        uint64_t kk = hkeys[s];
+        if (kk == key) return hvals[s];
+        if (kk == ~0ull) return ~0u;
+        s = (s + 1) & hmask;
+    }
+}
+
+// First index in [lo, hi) with keys[i] >= v.
+__device__ __forceinline__ uint32_t lower_bound_u64(const uint64_t *__restrict__ keys,
+                                                    uint32_t lo, uint32_t hi, uint64_t v) {
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (keys[mid] < v)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Decode the k cell coordinates of a linear id (inverse of S3).
+__device__ __forceinline__ void decode_cell(uint64_t key, const GridGeom &g, uint32_t *c) {
+    for (uint32_t j = 0; j < g.k; j++) {
+        uint64_t q = key / g.nCells[j];
+        c[j] = (uint32_t)(key - q * g.nCells[j]);
+        key = q;
+    }
+}
+
+// Stencil bounds of one indexed dim (kernel.cu:780-781).  With the cell-0
+// wrap the lower bound of coordinate 0 is UINT_MAX and the range is empty.
+__device__ __forceinline__ bool stencil_dim(uint32_t c, uint32_t ncell, uint32_t emulate,
+                                            uint32_t *lo, uint32_t *hi) {
+    uint32_t l;
+    if (c == 0) {
+        if (emulate) return false;
+        l = 0;
+    } else {
+        l = c - 1;
+    }
+    uint32_t h = c + 1 < ncell - 1 ? c + 1 : ncell - 1;
+    *lo = l;
+    *hi = h;
+    return l <= h;
+}
+
+// Enumerate the stencil rows of a cell: row r in [0, 3^(k-1)) maps to offsets
+// in dims 1..k-1.  Writes the candidate cell range [*cfirst, *clast) of row r
+// (cells with coordinate 0 in [lo0, hi0]), or an empty range.
+__device__ __forceinline__ void stencil_row(const uint32_t *c, const uint32_t *lo,
+                                            const uint32_t *hi, uint32_t r, const GridGeom &g,
+                                            const DevIndex &ix, uint32_t *cfirst,
+                                            uint32_t *clast) {
+    *cfirst = 0;
+    *clast = 0;
+    uint64_t rkey = 0, mult = 1;
+    for (uint32_t j = 1; j < g.k; j++) {
+        uint32_t off = r % 3;
+        r /= 3;
+        uint32_t cj = c[j] + off - 1;  // wraps for c[j] == 0, rejected below
+        if (cj < lo[j] || cj > hi[j]) return;
+        rkey += (uint64_t)cj * mult;
+        mult *= g.nCells[j];
+    }
+    uint32_t row = row_lookup(ix.hkeys, ix.hvals, ix.hmask, rkey);
+    if (row == ~0u) return;
+    uint32_t rs = ix.row_start[row], re = ix.row_start[row + 1];
+    uint64_t base = rkey * g.nCells[0];
+    uint32_t a = lower_bound_u64(ix.cell_key, rs, re, base + lo[0]);
+    uint32_t b = lower_bound_u64(ix.cell_key, a, re, base + hi[0] + 1);
+    *cfirst = a;
+    *clast = b;
+}
+
+__host__ __device__ inline uint32_t pow3(uint32_t e) {
+    uint32_t r = 1;
+    while (e--) r *= 3;
+    return r;
+}
+
+// Orders this wave's LDS writes before its later LDS reads by other lanes.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+}  // namespace hegj

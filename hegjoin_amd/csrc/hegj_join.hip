@@ -1,0 +1,404 @@
+// hegj_join.hip -- workload histogram and the FP64 epsilon-grid join.
+//
+// sortByWorkLoadGlobal (kernel.cu:74-144) becomes k_cell_workload, one thread
+// per non-empty cell walking the 3^(k-1) stencil *rows* (a row is a run of
+// cells that differ only in indexed coordinate 0, so the three dim-0
+// neighbours of a stencil position are one contiguous range of cells and of
+// points): one hash probe and two short binary searches per row instead of
+// two binary searches over all cells per stencil cell.
+//
+// kernelNDGridIndexGlobal / evaluateCell / evalPoint (kernel.cu:727-802,
+// 378-412, 333-370) become k_join: a persistent kernel in which each 64-lane
+// wavefront takes work units (a non-empty cell and up to QB of its points as
+// queries) from a device queue.  Per unit the wave
+//   1. enumerates the stencil rows, one row per lane, into a compacted LDS
+//      list of candidate point ranges (cell-order positions);
+//   2. streams the candidates 64 at a time: lane t loads candidate t's
+//      coordinates from the cell-ordered SoA copy (coalesced), evaluates the
+//      exact FP64 chain acc = fma(c-q, c-q, acc) in dimension order against
+//      every query of the unit (query coordinates are wave-uniform), and the
+//      predicate acc <= T (T = largest s with sqrt(s) <= eps, so this equals
+//      sqrt(acc) <= eps exactly); a ballot gives the 64-bit hit mask, which
+//      is kept in LDS, its popcount is the query's running count;
+//   3. reserves one contiguous segment for all its queries' neighbours with
+//      a single 64-bit atomic, then writes each query's neighbours (as
+//      original point ids) contiguously: lane positions come from popcounts
+//      of the kept masks, so no per-pair atomics and no sort are needed.  A
+//      unit that does not fit in the current result batch is deferred to the
+//      next batch (exact, no estimate can overflow a buffer).
+// The per-query (offset, count) table is the reference's neighborTable
+// (GPU.cu:1731-1776) with a single data array per batch.
+#include "hegj_internal.h"
+
+namespace hegj {
+
+
+namespace {
+
+constexpr int kMaxChunks = 64;   // hit masks kept in LDS per query
+constexpr int kWavesPerBlock = 4;
+
+// Compacted stencil-row list of one unit in LDS.
+struct RowList {
+    uint32_t start[kMaxRows];    // first candidate position of slot s
+    uint32_t off[kMaxRows + 1];  // prefix of slot lengths
+};
+
+// Enumerate the stencil rows of cell `cell` into rl; returns the number of
+// candidates M and the number of slots through *nslots.
+__device__ uint32_t build_rows(const JoinArgs &a, uint32_t cell, RowList &rl, uint32_t *nslots,
+                               bool *live) {
+    const GridGeom &g = a.g;
+    const int lane = threadIdx.x & 63;
+    uint32_t c[kMaxK], lo[kMaxK], hi[kMaxK];
+    decode_cell(a.ix.cell_key[cell], g, c);
+    bool ok = true;
+    for (uint32_t j = 0; j < g.k; j++) ok &= stencil_dim(c[j], g.nCells[j], g.emulate_cell0, &lo[j], &hi[j]);
+    *live = ok;
+    if (!ok) {
+        *nslots = 0;
+        return 0;
+    }
+    const uint32_t nrows = pow3(g.k - 1);
+    uint32_t slots = 0, total = 0;
+    for (uint32_t r0 = 0; r0 < nrows; r0 += 64) {
+        uint32_t r = r0 + lane;
+        uint32_t pstart = 0, plen = 0;
+        if (r < nrows) {
+            uint32_t cf, cl;
+            stencil_row(c, lo, hi, r, g, a.ix, &cf, &cl);
+            if (cl > cf) {
+                pstart = a.ix.cell_start[cf];
+                plen = a.ix.cell_start[cl] - pstart;
+            }
+        }
+        uint64_t nz = __ballot(plen > 0);
+        // inclusive scan of plen across the wave
+        uint32_t x = plen;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (plen > 0) {
+            uint32_t s = slots + lanes_below(nz);
+            rl.start[s] = pstart;
+            rl.off[s] = total + x - plen;
+        }
+        slots += (uint32_t)__popcll(nz);
+        total += __shfl(x, 63, 64);
+    }
+    if (lane == 0) rl.off[slots] = total;
+    *nslots = slots;
+    return total;
+}
+
+// Candidate position of flattened candidate index t (t < M).
+__device__ __forceinline__ uint32_t cand_pos(const RowList &rl, uint32_t nslots, uint32_t t) {
+    uint32_t lo = 0, hi = nslots;  // last slot with off <= t
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (rl.off[mid] <= t)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return rl.start[lo] + (t - rl.off[lo]);
+}
+
+// acc[qi] for one candidate chunk against the unit's queries.
+template <int D, int QB>
+__device__ __forceinline__ void chunk_dist(const JoinArgs &a, uint32_t pos, bool valid,
+                                           uint32_t qpos0, uint32_t nq, double *acc) {
+    const uint32_t dim = D > 0 ? (uint32_t)D : a.g.dim;
+    const uint64_t n = a.g.n;
+    const double *__restrict__ soa = a.ix.soa;
+    constexpr int DT = (D > 0 && D <= 32) ? D : 16;
+#pragma unroll
+    for (int qi = 0; qi < QB; qi++) acc[qi] = 0.0;
+    for (uint32_t d0 = 0; d0 < dim; d0 += DT) {
+        double x[DT];
+#pragma unroll
+        for (int i = 0; i < DT; i++) {
+            uint32_t l = d0 + i;
+            x[i] = (valid && l < dim) ? soa[(uint64_t)l * n + pos] : 0.0;
+        }
+#pragma unroll
+        for (int qi = 0; qi < QB; qi++) {
+            if ((uint32_t)qi < nq) {
+                const uint64_t qp = qpos0 + qi;
+#pragma unroll
+                for (int i = 0; i < DT; i++) {
+                    uint32_t l = d0 + i;
+                    if (l < dim) {
+                        double q = soa[(uint64_t)l * n + qp];
+                        double dd = x[i] - q;
+                        acc[qi] = __builtin_fma(dd, dd, acc[qi]);
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <int D, int QB>
+__global__ __launch_bounds__(64 * kWavesPerBlock) void k_join(JoinArgs a) {
+    __shared__ RowList rls[kWavesPerBlock];
+    __shared__ uint64_t masks[kWavesPerBlock][QB][kMaxChunks];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    RowList &rl = rls[wid];
+    const double T = a.g.thresh;
+    uint64_t my_hash = 0, my_evals = 0, my_pairs = 0;
+
+    while (true) {
+        uint32_t grab = 0;
+        if (lane == 0) grab = atomicAdd(a.queue_head, a.units_per_grab);
+        grab = wave_bcast_u32(grab);
+        if (grab >= a.n_units) break;
+        uint32_t gend = min(grab + a.units_per_grab, a.n_units);
+        for (uint32_t ui = grab; ui < gend; ui++) {
+            uint2 unit = a.units[ui];
+            const uint32_t cell = wave_bcast_u32(unit.x), qoff = wave_bcast_u32(unit.y);
+            const uint32_t cstart = a.ix.cell_start[cell];
+            const uint32_t csize = a.ix.cell_start[cell + 1] - cstart;
+            const uint32_t nq = min((uint32_t)QB, csize - qoff);
+            const uint32_t qpos0 = cstart + qoff;
+            uint32_t nslots;
+            bool live;
+            const uint32_t M = build_rows(a, cell, rl, &nslots, &live);
+            wave_lds_sync();
+            uint32_t cnt[QB];
+#pragma unroll
+            for (int qi = 0; qi < QB; qi++) cnt[qi] = 0;
+            const uint32_t nchunks = (M + 63) / 64;
+            for (uint32_t ch = 0; ch < nchunks; ch++) {
+                uint32_t t = ch * 64 + lane;
+                bool valid = t < M;
+                uint32_t pos = valid ? cand_pos(rl, nslots, t) : 0u;
+                double acc[QB];
+                chunk_dist<D, QB>(a, pos, valid, qpos0, nq, acc);
+#pragma unroll
+                for (int qi = 0; qi < QB; qi++) {
+                    if ((uint32_t)qi < nq) {
+                        uint64_t m = __ballot(valid && acc[qi] <= T);
+                        cnt[qi] += (uint32_t)__popcll(m);
+                        if (ch < kMaxChunks && lane == 0) masks[wid][qi][ch] = m;
+                    }
+                }
+            }
+            my_evals += (uint64_t)M * nq;
+            uint32_t total = 0, excl[QB];
+#pragma unroll
+            for (int qi = 0; qi < QB; qi++) {
+                excl[qi] = total;
+                total += cnt[qi];
+            }
+            if (a.count_only) {
+                my_pairs += total;
+                continue;
+            }
+            uint64_t base = 0;
+            if (total > 0) {
+                if (lane == 0) base = atomicAdd(a.cursor, (unsigned long long)total);
+                base = wave_bcast_u64(base);
+                if (base + total > a.capacity) {
+                    if (lane == 0) a.deferred[atomicAdd(a.n_deferred, 1u)] = ui;
+                    continue;
+                }
+            }
+            my_pairs += total;
+            wave_lds_sync();
+            // write phase
+#pragma unroll
+            for (int qi = 0; qi < QB; qi++) {
+                if ((uint32_t)qi >= nq) continue;
+                const uint32_t qid = a.ix.ila[qpos0 + qi];
+                if (lane == 0) {
+                    a.row_off[qid] = a.out_base + base + excl[qi];
+                    a.row_cnt[qid] = cnt[qi];
+                }
+                if (cnt[qi] == 0) continue;
+                uint64_t o = base + excl[qi];
+                for (uint32_t ch = 0; ch < nchunks; ch++) {
+                    uint64_t m;
+                    if (ch < kMaxChunks) {
+                        m = masks[wid][qi][ch];
+                    } else {
+                        // masks beyond the LDS budget are recomputed
+                        uint32_t t = ch * 64 + lane;
+                        bool valid = t < M;
+                        uint32_t pos = valid ? cand_pos(rl, nslots, t) : 0u;
+                        double acc[1];
+                        chunk_dist<D, 1>(a, pos, valid, qpos0 + qi, 1, acc);
+                        m = __ballot(valid && acc[0] <= T);
+                        my_evals += (uint64_t)min(64u, M - ch * 64);
+                    }
+                    if (m == 0) continue;
+                    if ((m >> lane) & 1ull) {
+                        uint32_t pos = cand_pos(rl, nslots, ch * 64 + lane);
+                        uint32_t id = a.ix.ila[pos];
+                        a.out[o + lanes_below(m)] = id;
+                        my_hash += mix64(((uint64_t)qid << 32) | id);
+                    }
+                    o += (uint32_t)__popcll(m);
+                }
+            }
+        }
+    }
+    // one atomic per wave for the counters
+    for (int o = 32; o > 0; o >>= 1) {
+        my_hash += __shfl_xor(my_hash, o, 64);
+    }
+    if (lane == 0) {
+        if (my_hash) atomicAdd(a.hash, (unsigned long long)my_hash);
+        if (my_evals) atomicAdd(a.evals, (unsigned long long)my_evals);
+        if (my_pairs) atomicAdd(a.pairs, (unsigned long long)my_pairs);
+    }
+}
+
+// Per-cell candidate count over the stencil (sortByWorkLoadGlobal).
+__global__ __launch_bounds__(256) void k_cell_workload(GridGeom g, DevIndex ix,
+                                                       uint32_t *__restrict__ work) {
+    uint32_t cell = blockIdx.x * 256 + threadIdx.x;
+    if (cell >= ix.C) return;
+    uint32_t c[kMaxK], lo[kMaxK], hi[kMaxK];
+    decode_cell(ix.cell_key[cell], g, c);
+    bool ok = true;
+    for (uint32_t j = 0; j < g.k; j++) ok &= stencil_dim(c[j], g.nCells[j], g.emulate_cell0, &lo[j], &hi[j]);
+    uint32_t total = 0;
+    if (ok) {
+        uint32_t nrows = pow3(g.k - 1);
+        for (uint32_t r = 0; r < nrows; r++) {
+            uint32_t cf, cl;
+            stencil_row(c, lo, hi, r, g, ix, &cf, &cl);
+            if (cl > cf) total += ix.cell_start[cl] - ix.cell_start[cf];
+        }
+    }
+    work[cell] = total;
+}
+
+// Units: cell c contributes ceil(|c| / qb) units when its stencil is live.
+__global__ __launch_bounds__(256) void k_unit_counts(GridGeom g, DevIndex ix, uint32_t qb,
+                                                     const uint32_t *__restrict__ order,
+                                                     uint32_t *__restrict__ ucount) {
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= ix.C) return;
+    uint32_t cell = order ? order[i] : i;
+    uint32_t c[kMaxK];
+    decode_cell(ix.cell_key[cell], g, c);
+    bool live = true;
+    for (uint32_t j = 0; j < g.k; j++)
+        if (c[j] == 0 && g.emulate_cell0) live = false;
+    uint32_t sz = ix.cell_start[cell + 1] - ix.cell_start[cell];
+    ucount[i] = live ? (sz + qb - 1) / qb : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_unit_fill(DevIndex ix, uint32_t qb,
+                                                   const uint32_t *__restrict__ order,
+                                                   const uint32_t *__restrict__ ucount,
+                                                   const uint32_t *__restrict__ uoff,
+                                                   uint2 *__restrict__ units) {
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= ix.C) return;
+    uint32_t cell = order ? order[i] : i;
+    for (uint32_t u = 0; u < ucount[i]; u++) units[uoff[i] + u] = make_uint2(cell, u * qb);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+hegj_status launch_workload(const GridGeom &g, const DevIndex &ix, uint32_t *work, hipStream_t s) {
+    if (ix.C == 0) return HEGJ_OK;
+    k_cell_workload<<<ceil_div(ix.C, 256), 256, 0, s>>>(g, ix, work);
+    HIP_TRY(hipGetLastError());
+    return HEGJ_OK;
+}
+
+hegj_status build_units(const GridGeom &g, const DevIndex &ix, uint32_t qb, const uint32_t *order,
+                        uint2 **units, uint32_t *n_units, hipStream_t s) {
+    Scratch sc;
+    uint32_t *ucount, *uoff, *d_total;
+    uint32_t C = ix.C;
+    HIP_TRY(sc.get(&ucount, C));
+    HIP_TRY(sc.get(&uoff, C));
+    HIP_TRY(sc.get(&d_total, 1));
+    k_unit_counts<<<ceil_div(C, 256), 256, 0, s>>>(g, ix, qb, order, ucount);
+    HIP_TRY(hipGetLastError());
+    HEGJ_TRY(scan_u32(ucount, uoff, C, d_total, s));
+    uint32_t U;
+    HIP_TRY(hipMemcpyAsync(&U, d_total, sizeof(U), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipMalloc(units, sizeof(uint2) * (U + 1)));
+    k_unit_fill<<<ceil_div(C, 256), 256, 0, s>>>(ix, qb, order, ucount, uoff, *units);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    *n_units = U;
+    return HEGJ_OK;
+}
+
+template <int D, int QB>
+static hegj_status launch_join_t(const JoinArgs &a, uint32_t blocks, hipStream_t s) {
+    k_join<D, QB><<<blocks, 64 * kWavesPerBlock, 0, s>>>(a);
+    HIP_TRY(hipGetLastError());
+    return HEGJ_OK;
+}
+
+template <int QB>
+static hegj_status launch_join_qb(const JoinArgs &a, uint32_t blocks, hipStream_t s) {
+    switch (a.g.dim) {
+        case 2: return launch_join_t<2, QB>(a, blocks, s);
+        case 6: return launch_join_t<6, QB>(a, blocks, s);
+        case 18: return launch_join_t<18, QB>(a, blocks, s);
+        case 90: return launch_join_t<90, QB>(a, blocks, s);
+        default: return launch_join_t<0, QB>(a, blocks, s);
+    }
+}
+
+hegj_status launch_join(const JoinArgs &a, uint32_t qb, uint32_t blocks, hipStream_t s) {
+    switch (qb) {
+        case 4: return launch_join_qb<4>(a, blocks, s);
+        case 8: return launch_join_qb<8>(a, blocks, s);
+        case 16: return launch_join_qb<16>(a, blocks, s);
+        default: return fail(HEGJ_ERR_INVALID_ARG, "queries_per_unit must be 4, 8 or 16");
+    }
+}
+
+}  // namespace hegj
+
+namespace hegj {
+
+hegj_status join_occupancy(uint32_t dim, uint32_t qb, int device, uint32_t *blocks) {
+    int per_cu = 0;
+    const void *fn = nullptr;
+#define HEGJ_OCC(D, Q) fn = reinterpret_cast<const void *>(&k_join<D, Q>)
+    uint32_t d = (dim == 2 || dim == 6 || dim == 18 || dim == 90) ? dim : 0;
+    switch (qb * 1000 + d) {
+        case 4002: HEGJ_OCC(2, 4); break;
+        case 4006: HEGJ_OCC(6, 4); break;
+        case 4018: HEGJ_OCC(18, 4); break;
+        case 4090: HEGJ_OCC(90, 4); break;
+        case 4000: HEGJ_OCC(0, 4); break;
+        case 8002: HEGJ_OCC(2, 8); break;
+        case 8006: HEGJ_OCC(6, 8); break;
+        case 8018: HEGJ_OCC(18, 8); break;
+        case 8090: HEGJ_OCC(90, 8); break;
+        case 8000: HEGJ_OCC(0, 8); break;
+        case 16002: HEGJ_OCC(2, 16); break;
+        case 16006: HEGJ_OCC(6, 16); break;
+        case 16018: HEGJ_OCC(18, 16); break;
+        case 16090: HEGJ_OCC(90, 16); break;
+        case 16000: HEGJ_OCC(0, 16); break;
+        default: return fail(HEGJ_ERR_INVALID_ARG, "queries_per_unit must be 4, 8 or 16");
+    }
+#undef HEGJ_OCC
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * kWavesPerBlock, 0));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    if (per_cu < 1) per_cu = 1;
+    *blocks = (uint32_t)per_cu * (uint32_t)prop.multiProcessorCount;
+    return HEGJ_OK;
+}
+
+}  // namespace hegj

@@ -1,0 +1,194 @@
+// main.cpp -- `./main dataset epsilon dimensionality algorithm [static_partitioning]`
+//
+// Drop-in for the reference CLI (main.cu:42-446, README.md:57-74) for
+// algorithm 0 (GPU only).  Argument checks and exit codes follow
+// main.cu:47-71; the [RESULT] lines follow main.cu:366-407, GPU.cu:1622-1628
+// and SortByWorkload.cu:220-221 so `| grep RESULT` pipelines keep working.
+// Algorithms 1-3 (hybrid CPU+GPU, CPU only) are not part of this build and
+// are rejected with exit code 1.
+//
+// Compile-time parameters keep the params.h semantics: -DGPUNUMDIM=<d>
+// makes the CLI insist that dim == GPUNUMDIM (main.cu:61-65);
+// -DNUMINDEXEDDIM=<k> sets the indexed dimensions (default 6, capped at dim).
+// Extra flags after the positional arguments:
+//   --indexed-dims=K    override NUMINDEXEDDIM at run time
+//   --no-cell0-wrap     do not reproduce the reference's cell-0 wrap
+//   --dump-csr=PATH     write the neighbour table (rows sorted) as binary:
+//                       uint32 n, uint64 offsets[n+1], uint32 neighbours[]
+//   --device=D          HIP device ordinal
+#include <cerrno>
+#include <chrono>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "hegjoin.h"
+
+#ifndef NUMINDEXEDDIM
+#define NUMINDEXEDDIM 6
+#endif
+
+#define NB_ARGS_MAX 6
+
+static double now_s() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+int main(int argc, char **argv) {
+    double tStartStart = now_s();
+    std::printf("\n\n========== HEGJoin (MI355X) ==========\n\n\n");
+    std::vector<char *> pos;
+    uint32_t kOverride = 0;
+    int emulate = 1, device = 0;
+    const char *dump = nullptr;
+    for (int i = 1; i < argc; i++) {
+        if (std::strncmp(argv[i], "--indexed-dims=", 15) == 0)
+            kOverride = (uint32_t)std::atoi(argv[i] + 15);
+        else if (std::strcmp(argv[i], "--no-cell0-wrap") == 0)
+            emulate = 0;
+        else if (std::strncmp(argv[i], "--dump-csr=", 11) == 0)
+            dump = argv[i] + 11;
+        else if (std::strncmp(argv[i], "--device=", 9) == 0)
+            device = std::atoi(argv[i] + 9);
+        else
+            pos.push_back(argv[i]);
+    }
+    int nargs = (int)pos.size() + 1;
+    if (NB_ARGS_MAX != nargs && (NB_ARGS_MAX - 1) != nargs) {
+        std::fprintf(stderr, "[MAIN] ~ Expected %d or %d args, found %d\n", NB_ARGS_MAX, NB_ARGS_MAX - 1, nargs);
+        std::fprintf(stderr, "[MAIN] ~ Args: filename epsilon dim searchmode [CPU-GPU partitioning]\n");
+        return 1;
+    }
+    const char *filename = pos[0];
+    double epsilon = std::atof(pos[1]);
+    int dim = std::atoi(pos[2]);
+    int searchMode = std::atoi(pos[3]);
+#ifdef GPUNUMDIM
+    if (GPUNUMDIM != dim) {
+        std::fprintf(stderr, "[MAIN] ~ Error: dim should be equals to GPUNUMDIM (see params.h)\n");
+        return 1;
+    }
+#endif
+    if (dim <= 0) {
+        std::fprintf(stderr, "[MAIN] ~ Error: dim must be positive\n");
+        return 1;
+    }
+    if (epsilon <= 0.0 || 1.0 < epsilon) {
+        std::fprintf(stderr, "[MAIN] ~ Error: epsilon should be between ]0.0, 1.0]\n");
+        return 1;
+    }
+    std::printf("[MAIN] ~ Dataset: %s\n", filename);
+    std::printf("[MAIN] ~ Epsilon: %f\n", epsilon);
+    std::printf("[MAIN] ~ Dimensionality: %d\n", dim);
+    std::printf("[MAIN] ~ Search mode: %d\n", searchMode);
+    if (searchMode != 0) {
+        std::fprintf(stderr,
+                     "[MAIN] ~ Error: this build provides algorithm 0 (GPU only); "
+                     "algorithms 1-3 (hybrid / CPU Super-EGO) are not built\n");
+        return 1;
+    }
+
+    double t0 = now_s();
+    double *data = nullptr;
+    uint32_t n = 0;
+    if (hegj_dataset_load(filename, (uint32_t)dim, &data, &n) != HEGJ_OK) {
+        std::fprintf(stderr, "[MAIN] ~ Error: %s\n", hegj_last_error());
+        return 1;
+    }
+    std::printf("[MAIN] ~ Time to read the dataset: %f\n", now_s() - t0);
+    std::printf("Data import: Total data points: %u\n", n);
+    if (n == 0) {
+        std::fprintf(stderr, "[MAIN] ~ Error: empty dataset\n");
+        return 1;
+    }
+
+    hegj_index_opts o;
+    hegj_index_opts_default(&o, (uint32_t)dim);
+    uint32_t k = kOverride ? kOverride : (uint32_t)NUMINDEXEDDIM;
+    o.indexed_dims = k < (uint32_t)dim ? k : (uint32_t)dim;
+    o.emulate_cell0 = (uint32_t)emulate;
+    o.device = device;
+
+    double tIndex = now_s();
+    hegj_index *idx = nullptr;
+    hegj_status st = hegj_index_build(data, n, epsilon, &o, &idx);
+    hegj_dataset_free(data);
+    if (st != HEGJ_OK) {
+        std::fprintf(stderr, "[INDEX] ~ Error: %s: %s\n", hegj_status_string(st), hegj_last_error());
+        return 1;
+    }
+    hegj_index_info ii;
+    hegj_index_get_info(idx, &ii);
+    std::printf("[MAIN] ~ Total cells (including empty): %lu\n", (unsigned long)ii.totalCells);
+    std::printf("[INDEX] ~ Number of full cells (non-empty): %u\n", ii.nNonEmptyCells);
+    std::printf("[INDEX] ~ Queries silenced by the cell-0 wrap: %u\n", ii.n_cell0_queries);
+    std::printf("[INDEX] ~ Time to index using the GPU (including allocating and transfering memory): %f\n",
+                now_s() - tIndex);
+
+    double tStartSort = now_s();
+    uint64_t totalCandidates = 0;
+    std::vector<hegj_scheduling_cell> cells(ii.nNonEmptyCells);
+    st = hegj_sort_by_workload(idx, cells.data(), nullptr, &totalCandidates);
+    if (st != HEGJ_OK) {
+        std::fprintf(stderr, "[SORT] ~ Error: %s: %s\n", hegj_status_string(st), hegj_last_error());
+        return 1;
+    }
+    double sortTime = now_s() - tStartSort;
+    if (!cells.empty())
+        std::printf("[SORT | RESULT] ~ Total number of candidate points to refine: %llu\n"
+                    "[SORT | RESULT] ~ Number of candidates: min = %d, median = %d, max = %d, avg = %llu\n",
+                    (unsigned long long)totalCandidates, cells.back().nbPoints,
+                    cells[cells.size() / 2].nbPoints, cells.front().nbPoints,
+                    (unsigned long long)(totalCandidates / n));
+    std::printf("\n\n[MAIN] ~ Time to do everything before computing: %f\n\n\n", now_s() - tStartStart);
+
+    double tBeginGPU = now_s();
+    hegj_result *res = nullptr;
+    st = hegj_selfjoin(idx, nullptr, &res);
+    if (st != HEGJ_OK) {
+        std::fprintf(stderr, "[GPU] ~ Error: %s: %s\n", hegj_status_string(st), hegj_last_error());
+        return 1;
+    }
+    double gpuTime = now_s() - tBeginGPU;
+    hegj_result_info ri;
+    hegj_result_get_info(res, &ri);
+    std::printf("[GPU] ~ Kernel execution time: %f ms over %u launch(es), %u batch(es)\n",
+                ri.join_ms, ri.join_launches, ri.n_batches);
+    std::printf("[GPU | RESULT] ~ Query points computed by the GPU: %u (f: %f)\n", n, 1.0);
+    std::printf("[GPU | RESULT] ~ Compute time for the GPU: %f\n", gpuTime);
+
+    std::printf("[RESULT] ~ Total result set size: %lu\n", (unsigned long)ri.total_neighbors);
+    std::printf("   [RESULT] ~ Total result set size on the GPU: %lu\n", (unsigned long)ri.total_neighbors);
+    std::printf("   [RESULT] ~ Total result set size on the CPU: %lu\n", 0ul);
+    std::printf("[RESULT] ~ Total execution time: %f\n", gpuTime + sortTime);
+    std::printf("   [RESULT] ~ Total execution time to SortByWL: %f\n", sortTime);
+    std::printf("   [RESULT] ~ Total execution time for the GPU: %f\n", gpuTime);
+    std::printf("   [RESULT] ~ Total execution time for the CPU: %f (reorder: %f, sort: %f, total = %f)\n",
+                0.0, 0.0, 0.0, 0.0);
+    std::printf("[RESULT] ~ Pair multiset hash: %016llx\n", (unsigned long long)ri.pair_hash);
+
+    int rc = 0;
+    if (dump) {
+        std::vector<uint64_t> off(n + 1);
+        std::vector<uint32_t> nb(ri.total_neighbors);
+        st = hegj_result_copy_csr(res, off.data(), nb.data(), 1);
+        FILE *f = st == HEGJ_OK ? std::fopen(dump, "wb") : nullptr;
+        if (!f) {
+            std::fprintf(stderr, "[MAIN] ~ Error: cannot write %s\n", dump);
+            rc = 1;
+        } else {
+            std::fwrite(&n, 4, 1, f);
+            std::fwrite(off.data(), 8, off.size(), f);
+            std::fwrite(nb.data(), 4, nb.size(), f);
+            std::fclose(f);
+        }
+    }
+    hegj_result_free(res);
+    hegj_index_free(idx);
+    return rc;
+}

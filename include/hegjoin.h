@@ -1,0 +1,205 @@
+/*
+ * hegjoin.h -- C ABI of the MI355X-native epsilon-grid self-join
+ * (libhegjoin.so).  Plain pointers and sizes only; no HIP or torch types.
+ *
+ * It replaces the three host entry points of the reference's algorithm 0
+ * (the GPU-only mode of `./main dataset epsilon dim algorithm`):
+ *
+ *   hegj_index_build      <- generateNDGridDimensions  (main.cu:449-520)
+ *                          + gridIndexingGPU           (GPU.h:9-27, GPU.cu:74-398)
+ *   hegj_sort_by_workload <- sortByWorkLoad            (SortByWorkload.h:7-27,
+ *                                                        SortByWorkload.cu:19-247)
+ *   hegj_selfjoin         <- distanceTableNDGridBatches (GPU.h:29-54,
+ *                                                        GPU.cu:787-1702)
+ *                          + GPUBatchEst_v2            (GPU.h:73-88, GPU.cu:403-779)
+ *   hegj_result_*         <- constructNeighborTableKeyValueWithPtrs
+ *                                                       (GPU.h:90-95, GPU.cu:1731-1776)
+ *
+ * Semantics kept from the reference (params.h, structs.h):
+ *   - GPUNUMDIM   -> hegj_index_opts.dim, NUMINDEXEDDIM -> .indexed_dims,
+ *     DTYPE       -> .dtype (HEGJ_DTYPE_F64 is DTYPE=double);
+ *   - the grid-cell index layouts: hegj_grid == struct grid,
+ *     hegj_grid_cell_lookup == struct gridCellLookup, hegj_scheduling_cell ==
+ *     struct schedulingCell (structs.h:83-104), byte for byte; the
+ *     neighbour table (struct neighborTableLookup rows over per-batch arrays,
+ *     structs.h:35-41) is exported as CSR by point id;
+ *   - the pair set: ordered pairs (query, candidate) with
+ *     sqrt(fma-chain distance) <= eps over the 3^k cell stencil, self-pairs
+ *     included, and (by default) the reference's cell-0 wrap, which silences
+ *     queries whose cell has coordinate 0 in an indexed dimension.
+ *
+ * Conventions: every call returns hegj_status (no exit, no printing);
+ * hegj_last_error() gives a message for the calling thread.  The library owns
+ * device memory; host buffers are caller-owned.  A handle is used by one
+ * thread at a time.  Each handle has its own HIP stream on its device.
+ */
+#ifndef HEGJOIN_H
+#define HEGJOIN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HEGJ_MAX_INDEXED_DIMS 6
+
+typedef enum {
+    HEGJ_OK = 0,
+    HEGJ_ERR_INVALID_ARG = 1,
+    HEGJ_ERR_UNSUPPORTED = 2,
+    HEGJ_ERR_NO_DEVICE = 3,
+    HEGJ_ERR_OUT_OF_MEMORY = 4,
+    HEGJ_ERR_HIP = 5,
+    HEGJ_ERR_CAPACITY = 6,
+    HEGJ_ERR_IO = 7,
+    HEGJ_ERR_BUSY = 8
+} hegj_status;
+
+typedef enum { HEGJ_DTYPE_F64 = 0, HEGJ_DTYPE_F32 = 1 } hegj_dtype;
+
+/* structs.h:83-86 */
+typedef struct { int32_t indexmin; int32_t indexmax; } hegj_grid;
+/* structs.h:90-98 (16 bytes: idx, 4 bytes padding, gridLinearID) */
+typedef struct { uint32_t idx; uint32_t _pad; uint64_t gridLinearID; } hegj_grid_cell_lookup;
+/* structs.h:101-104 */
+typedef struct { int32_t nbPoints; int32_t cellId; } hegj_scheduling_cell;
+
+/* ------------------------------------------------------------------------ */
+const char *hegj_status_string(hegj_status s);
+const char *hegj_last_error(void);
+/* Library version string. */
+const char *hegj_version(void);
+/* Number of visible HIP devices (0 when none). */
+int hegj_device_count(void);
+
+/* ------------------------------------------------------------------------ */
+/* Datasets.                                                                */
+/* ------------------------------------------------------------------------ */
+/* The reference generator (datasets/dataset_gen_fixed_len_expo_dist.cpp:57-84):
+ * mt19937(seed), exponential(lambda), redraw outside [0, 1]; out[n*dim]
+ * row-major.  seed 2137834274 reproduces the reference's own datasets. */
+hegj_status hegj_dataset_generate_expo(uint32_t seed, uint32_t n, uint32_t dim, double lambda,
+                                       double *out);
+/* importNDDataset (import_dataset.cpp:10-40): comma/whitespace separated
+ * numbers, n = values / dim; *.bin = raw row-major doubles.  *out is freed
+ * with hegj_dataset_free. */
+hegj_status hegj_dataset_load(const char *path, uint32_t dim, double **out, uint32_t *n);
+void hegj_dataset_free(double *p);
+
+/* ------------------------------------------------------------------------ */
+/* Grid index (generateNDGridDimensions + gridIndexingGPU).                  */
+/* ------------------------------------------------------------------------ */
+typedef struct hegj_index hegj_index;
+
+typedef struct {
+    uint32_t dim;           /* GPUNUMDIM (params.h:5) */
+    uint32_t indexed_dims;  /* NUMINDEXEDDIM (params.h:8), 1..HEGJ_MAX_INDEXED_DIMS */
+    uint32_t dtype;         /* hegj_dtype; DTYPE (params.h:24) */
+    uint32_t emulate_cell0; /* 1 = reproduce the reference's cell-0 wrap (kernel.cu:780) */
+    int32_t device;         /* HIP device ordinal */
+} hegj_index_opts;
+
+/* Fills opts with the reference defaults for a given dim (NUMINDEXEDDIM =
+ * min(dim, 6), DTYPE double, cell-0 wrap emulated, device 0). */
+void hegj_index_opts_default(hegj_index_opts *opts, uint32_t dim);
+
+/* database: host, row-major n x dim doubles (main.cu:120-128 `database`).
+ * epsilon must be in ]0, 1] (main.cu:67). */
+hegj_status hegj_index_build(const double *database, uint32_t n, double epsilon,
+                             const hegj_index_opts *opts, hegj_index **out);
+/* Same, from a device pointer on opts->device (data already resident). */
+hegj_status hegj_index_build_device(const double *dev_database, uint32_t n, double epsilon,
+                                    const hegj_index_opts *opts, hegj_index **out);
+hegj_status hegj_index_free(hegj_index *idx);
+
+typedef struct {
+    uint32_t n, dim, indexed_dims, dtype;
+    uint32_t nNonEmptyCells;
+    uint32_t n_rows;            /* cells grouped by indexed dims 1..k-1 */
+    uint32_t n_cell0_queries;   /* queries silenced by the cell-0 wrap */
+    uint32_t emulate_cell0;
+    uint64_t totalCells;
+    double epsilon;
+    double threshold;           /* largest s with sqrt(s) <= epsilon */
+    double minArr[HEGJ_MAX_INDEXED_DIMS];
+    double maxArr[HEGJ_MAX_INDEXED_DIMS];
+    uint32_t nCells[HEGJ_MAX_INDEXED_DIMS];
+    double build_ms;            /* device time of the index build */
+} hegj_index_info;
+hegj_status hegj_index_get_info(const hegj_index *idx, hegj_index_info *info);
+
+/* Host copies in the reference layouts (GPU.cu:264-324): index[C],
+ * gridCellLookupArr[C], indexLookupArr[n].  Any pointer may be NULL. */
+hegj_status hegj_index_copy_grid(const hegj_index *idx, hegj_grid *index,
+                                 hegj_grid_cell_lookup *gridCellLookupArr,
+                                 uint32_t *indexLookupArr);
+/* Per-point linear cell ids in original point order (kernel.cu:34-66). */
+hegj_status hegj_index_copy_cell_ids(const hegj_index *idx, uint64_t *ids);
+
+/* ------------------------------------------------------------------------ */
+/* sortByWorkLoad.                                                          */
+/* ------------------------------------------------------------------------ */
+/* sortedCells[C] (by candidate count descending, ties by cell id
+ * ascending), originPointIndex[n] (the member lists in that order) and
+ * totalCandidates (sum |cell| * candidates(cell), 64-bit).  Outputs may be
+ * NULL.  Also keeps the order on the device for the join. */
+hegj_status hegj_sort_by_workload(hegj_index *idx, hegj_scheduling_cell *sortedCells,
+                                  uint32_t *originPointIndex, uint64_t *totalCandidates);
+
+/* ------------------------------------------------------------------------ */
+/* Self-join (distanceTableNDGridBatches + neighbour table).                */
+/* ------------------------------------------------------------------------ */
+typedef struct hegj_result hegj_result;
+
+typedef enum {
+    HEGJ_ORDER_SPATIAL = 0,   /* work units in cell (linear id) order */
+    HEGJ_ORDER_WORKLOAD = 1   /* work units by descending cell workload */
+} hegj_unit_order;
+
+typedef struct {
+    uint64_t result_capacity; /* pairs per device result batch; 0 = estimate */
+    uint32_t unit_order;      /* hegj_unit_order */
+    uint32_t queries_per_unit;/* 0 = tuned default */
+    uint32_t block_size;      /* threads per workgroup, 0 = tuned default */
+    uint32_t reserved[3];
+} hegj_join_opts;
+
+void hegj_join_opts_default(hegj_join_opts *opts);
+
+hegj_status hegj_selfjoin(hegj_index *idx, const hegj_join_opts *opts, hegj_result **out);
+hegj_status hegj_result_free(hegj_result *res);
+
+typedef struct {
+    uint64_t total_neighbors;  /* [RESULT] Total result set size (main.cu:368) */
+    uint64_t pair_hash;        /* sum of mix64(query<<32 | neighbour) over all pairs */
+    uint64_t total_candidates; /* the reference's candidate count (SortByWorkload.cu:160) */
+    uint64_t evals_executed;   /* distance evaluations the kernels performed */
+    uint64_t band_rechecks;    /* pairs re-evaluated in FP64 inside the guard band */
+    uint32_t n_queries;        /* queries with a stencil (not silenced) */
+    uint32_t n_batches;        /* result batches used */
+    uint32_t n_units;          /* work units */
+    uint32_t n_deferred;       /* units re-run because a batch filled up */
+    double estimate_ms;        /* device time: batch estimator */
+    double join_ms;            /* device time: join kernel launches */
+    double workload_ms;        /* device time: workload sort */
+    double total_ms;           /* host wall time of hegj_selfjoin */
+    double join_kernel_avg_ms; /* average duration of one join kernel launch */
+    uint32_t join_launches;
+    uint32_t reserved;
+} hegj_result_info;
+hegj_status hegj_result_get_info(const hegj_result *res, hegj_result_info *info);
+
+/* Per-query neighbour counts, original point ids (neighborTable row sizes). */
+hegj_status hegj_result_copy_counts(const hegj_result *res, uint32_t *counts);
+/* The neighbour table as CSR by original point id: offsets[n+1],
+ * neighbors[total].  Within a row neighbours come in cell order; rows are
+ * sorted ascending when sort_rows != 0. */
+hegj_status hegj_result_copy_csr(const hegj_result *res, uint64_t *offsets, uint32_t *neighbors,
+                                 int sort_rows);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HEGJOIN_H */

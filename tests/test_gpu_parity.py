@@ -1,0 +1,151 @@
+"""GPU parity: libhegjoin (HIP, through the C ABI) against the oracle and the
+golden fixtures.  Integer outputs (cell ids, index layout, workload order,
+neighbour sets, counts, hashes) must be bit-identical."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import golden_files, load_golden, ROOT
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def H():
+    import hegjoin_amd as H
+    if H.device_count() == 0:
+        pytest.fail("no HIP device visible for a GPU test")
+    return H
+
+
+def gpu_join(H, pts, eps, k, emulate=True, **kw):
+    with H.GridIndex(pts, eps, indexed_dims=k, emulate_cell0=emulate) as ix:
+        t = ix.selfjoin(**kw)
+        off, nb = t.csr(sort_rows=True)
+        info = t.info
+        counts = t.counts()
+        t.close()
+    assert np.array_equal(np.diff(off).astype(np.uint32), counts)
+    return off, nb, info
+
+
+ALL = golden_files("alg0_") + golden_files("cell0_")
+
+
+@pytest.mark.parametrize("path", ALL, ids=os.path.basename)
+def test_index_matches_golden(H, path):
+    g = load_golden(path)
+    pts, k, eps = g["pts"], int(g["k"]), float(g["eps"])
+    print("case", os.path.basename(path), pts.shape, flush=True)
+    with H.GridIndex(pts, eps, indexed_dims=k) as ix:
+        info = ix.info
+        assert info["minArr"] == g["minArr"].tolist()
+        assert info["maxArr"] == g["maxArr"].tolist()
+        assert info["nCells"] == g["nCells"].tolist()
+        assert info["totalCells"] == int(g["totalCells"])
+        assert info["threshold"] == float(g["threshold"])
+        assert info["n_cell0_queries"] == int(g["n_cell0"])
+        assert np.array_equal(ix.cell_ids(), g["cell_ids"])
+        grid, lookup, ila = ix.copy_grid()
+        assert np.array_equal(ila, g["indexLookupArr"])
+        assert np.array_equal(lookup["gridLinearID"], g["cell_keys"])
+        assert np.array_equal(lookup["idx"], np.arange(len(lookup)))
+        assert np.array_equal(grid["indexmin"], g["grid_min"])
+        assert np.array_equal(grid["indexmax"], g["grid_max"])
+
+
+@pytest.mark.parametrize("path", ALL, ids=os.path.basename)
+def test_workload_matches_golden(H, path):
+    g = load_golden(path)
+    pts, k, eps = g["pts"], int(g["k"]), float(g["eps"])
+    with H.GridIndex(pts, eps, indexed_dims=k) as ix:
+        cells, opi, tot = ix.sort_by_workload()
+    assert tot == int(g["totalCandidates"])
+    assert np.array_equal(cells["cellId"], g["sorted_cells"].astype(np.int32))
+    assert np.array_equal(cells["nbPoints"], g["work"][g["sorted_cells"]].astype(np.int32))
+    assert np.array_equal(opi, g["originPointIndex"])
+
+
+@pytest.mark.parametrize("path", ALL, ids=os.path.basename)
+def test_join_matches_golden(H, path):
+    g = load_golden(path)
+    pts, k, eps = g["pts"], int(g["k"]), float(g["eps"])
+    off, nb, info = gpu_join(H, pts, eps, k)
+    assert info["total_neighbors"] == len(g["nbrs"])
+    assert np.array_equal(off, g["offsets"])
+    assert np.array_equal(nb, g["nbrs"])
+    assert info["pair_hash"] == int(g["hash"])
+    assert info["total_candidates"] == int(g["totalCandidates"])
+
+
+@pytest.mark.parametrize("path", golden_files("cell0_"), ids=os.path.basename)
+def test_join_without_cell0_wrap(H, path):
+    g = load_golden(path)
+    off, nb, info = gpu_join(H, g["pts"], float(g["eps"]), int(g["k"]), emulate=False)
+    assert np.array_equal(off, g["nowrap_offsets"]) and np.array_equal(nb, g["nowrap_nbrs"])
+
+
+@pytest.mark.parametrize("qb", [4, 8, 16])
+@pytest.mark.parametrize("order", ["spatial", "workload"])
+def test_join_variants_identical(H, qb, order):
+    g = load_golden(golden_files("cell0_d2")[0])
+    off, nb, info = gpu_join(H, g["pts"], float(g["eps"]), int(g["k"]), queries_per_unit=qb,
+                             unit_order=order)
+    assert np.array_equal(off, g["offsets"]) and np.array_equal(nb, g["nbrs"])
+    assert info["pair_hash"] == int(g["hash"])
+
+
+@pytest.mark.parametrize("cap", [5000, 40000])
+def test_small_batches_defer_and_match(H, cap):
+    """A result batch far smaller than the output forces many batches and
+    deferred units; the neighbour table must not change."""
+    g = load_golden(golden_files("alg0_d6_")[0])
+    off, nb, info = gpu_join(H, g["pts"], float(g["eps"]), int(g["k"]), result_capacity=cap)
+    assert info["n_batches"] > 1 and info["n_deferred"] > 0
+    assert np.array_equal(off, g["offsets"]) and np.array_equal(nb, g["nbrs"])
+    assert info["pair_hash"] == int(g["hash"])
+
+
+@pytest.mark.parametrize("dim,k,n,eps,seed", [
+    (1, 1, 5000, 0.0005, 1), (2, 1, 5000, 0.003, 2), (3, 3, 6000, 0.01, 3), (4, 2, 5000, 0.02, 4),
+    (5, 5, 5000, 0.03, 5), (7, 6, 4000, 0.05, 6), (8, 4, 4000, 0.05, 7), (6, 6, 20000, 0.012, 8),
+    (18, 3, 2000, 0.1, 9), (90, 2, 800, 0.4, 10), (33, 6, 1000, 0.2, 11), (2, 2, 50000, 0.001, 12)])
+def test_join_random_vs_oracle(H, oracle, dim, k, n, eps, seed):
+    """Generic-D kernels (dims without a specialised template) and
+    k < dim, against the oracle computed live."""
+    pts = oracle.generate_expo(n, dim, seed=seed)
+    print("case dim", dim, "k", k, "n", n, flush=True)
+    off, nb, info = gpu_join(H, pts, eps, k)
+    eoff, enb = oracle.join(pts, k, eps)
+    assert np.array_equal(off, eoff) and np.array_equal(nb, enb)
+    assert info["pair_hash"] == oracle.csr_hash(eoff, enb)
+
+
+def test_duplicate_points_and_tiny_inputs(H, oracle):
+    """Duplicates (distance 0), a single point, and all points in one cell."""
+    for pts, eps in [(np.full((50, 3), 0.5), 0.1), (np.array([[0.3, 0.7]]), 0.2),
+                     (np.random.default_rng(0).random((300, 2)) * 0.01 + 0.5, 0.5)]:
+        off, nb, _ = gpu_join(H, pts, eps, pts.shape[1])
+        eoff, enb = oracle.join(pts, pts.shape[1], eps)
+        assert np.array_equal(off, eoff) and np.array_equal(nb, enb)
+
+
+def test_cli_end_to_end(H, oracle, tmp_path):
+    import subprocess
+    cli = os.path.join(ROOT, "hegjoin_amd", "main")
+    pts = oracle.generate_expo(20000, 2, seed=5)
+    path = str(tmp_path / "d.txt")
+    oracle.write_text(path, pts, 17)
+    dump = str(tmp_path / "csr.bin")
+    r = subprocess.run([cli, path, "0.002", "2", "0", f"--dump-csr={dump}"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    eoff, enb = oracle.join(pts, 2, 0.002)
+    line = [l for l in r.stdout.splitlines() if l.startswith("[RESULT] ~ Total result set size:")]
+    assert int(line[0].split(":")[1]) == len(enb)
+    raw = open(dump, "rb").read()
+    n = int(np.frombuffer(raw[:4], np.uint32)[0])
+    off = np.frombuffer(raw[4:4 + 8 * (n + 1)], np.uint64)
+    nb = np.frombuffer(raw[4 + 8 * (n + 1):], np.uint32)
+    assert np.array_equal(off, eoff) and np.array_equal(nb, enb)
