@@ -27,7 +27,7 @@ LIB_PATH = os.path.join(_PKG, "libhegjoin.so")
 MAX_INDEXED_DIMS = 6
 
 if not os.path.exists(LIB_PATH):
-    raise ImportError(f"{LIB_PATH} is not built; run `python -m hegjoin_amd.build`")
+    raise ImportError(f"{LIB_PATH} is not built; run `python hegjoin_amd/build.py`")
 _lib = C.CDLL(LIB_PATH)
 
 GRID_DTYPE = np.dtype([("indexmin", np.int32), ("indexmax", np.int32)])
@@ -61,8 +61,9 @@ class _IndexInfo(C.Structure):
 
 class _JoinOpts(C.Structure):
     _fields_ = [("result_capacity", C.c_uint64), ("unit_order", C.c_uint32),
-                ("queries_per_unit", C.c_uint32), ("block_size", C.c_uint32),
-                ("reserved", C.c_uint32 * 3)]
+                ("queries_per_unit", C.c_uint32), ("refine", C.c_uint32),
+                ("xcd_parts", C.c_uint32), ("units_per_grab", C.c_uint32),
+                ("part_by_block", C.c_uint32)]
 
 
 class _ResultInfo(C.Structure):
@@ -95,6 +96,7 @@ _lib.hegj_dataset_generate_expo.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, 
 _lib.hegj_dataset_load.argtypes = [C.c_char_p, C.c_uint32, C.POINTER(C.POINTER(C.c_double)),
                                    C.POINTER(C.c_uint32)]
 _lib.hegj_dataset_free.argtypes = [_vp]
+_lib.hegj_stream_copy_bandwidth.argtypes = [C.c_uint64, C.c_int, C.POINTER(C.c_double)]
 _lib.hegj_index_opts_default.argtypes = [C.POINTER(_IndexOpts), C.c_uint32]
 _lib.hegj_index_build.argtypes = [_vp, C.c_uint32, C.c_double, C.POINTER(_IndexOpts), C.POINTER(_vp)]
 _lib.hegj_index_build_device.argtypes = _lib.hegj_index_build.argtypes
@@ -122,6 +124,13 @@ def version() -> str:
 
 def device_count() -> int:
     return _lib.hegj_device_count()
+
+
+def stream_copy_bandwidth(nbytes=1 << 30, device=0):
+    """Attainable HBM GB/s of a plain 16-byte-vector copy (read + write)."""
+    g = C.c_double()
+    _check(_lib.hegj_stream_copy_bandwidth(nbytes, device, C.byref(g)))
+    return g.value
 
 
 def library():
@@ -235,12 +244,21 @@ class GridIndex:
                                           C.byref(tot)))
         return cells, opi, tot.value
 
-    def selfjoin(self, result_capacity=0, unit_order="spatial", queries_per_unit=0):
+    def selfjoin(self, result_capacity=0, unit_order="morton", queries_per_unit=0, xcd_parts=0,
+                 refine="auto", units_per_grab=0, part_by_block=0):
+        """unit_order: "morton" (default), "spatial" (linear cell id) or
+        "workload" (descending candidates, the reference's order);
+        refine: "auto", "fp64" or "mfma" (split-BF16 MFMA + guard band +
+        FP64 band recheck); xcd_parts: unit ranges, one per XCD (0 = auto)."""
         o = _JoinOpts()
         _lib.hegj_join_opts_default(C.byref(o))
         o.result_capacity = result_capacity
-        o.unit_order = {"spatial": 0, "workload": 1}[unit_order]
+        o.unit_order = {"spatial": 0, "workload": 1, "morton": 2}[unit_order]
         o.queries_per_unit = queries_per_unit
+        o.xcd_parts = xcd_parts
+        o.refine = {"auto": 0, "fp64": 1, "mfma": 2}[refine]
+        o.units_per_grab = units_per_grab
+        o.part_by_block = part_by_block
         r = _vp()
         _check(_lib.hegj_selfjoin(self._h, C.byref(o), C.byref(r)))
         return NeighborTable(r, self)

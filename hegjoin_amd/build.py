@@ -1,7 +1,7 @@
 """Builds libhegjoin.so and the `main` CLI for gfx950 with hipcc, in-tree.
 
-    python -m hegjoin_amd.build            # library + CLI
-    python -m hegjoin_amd.build --params "-DGPUNUMDIM=6 -DNUMINDEXEDDIM=6"
+    python hegjoin_amd/build.py            # library + CLI
+    python hegjoin_amd/build.py --params "-DGPUNUMDIM=6 -DNUMINDEXEDDIM=6"
 
 Objects go to hegjoin_amd/_build/, the library to hegjoin_amd/libhegjoin.so,
 the CLI to hegjoin_amd/main (rpath $ORIGIN).  PARAMS mirrors the reference

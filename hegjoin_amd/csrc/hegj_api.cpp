@@ -84,6 +84,8 @@ struct hegj_index {
     // workload sort (lazily computed)
     uint32_t *d_work = nullptr;   // [C] candidates per cell
     uint32_t *d_order = nullptr;  // [C] cells by workload
+    uint32_t *d_morton = nullptr; // [C] cells in Morton order (lazily)
+    bool morton_ok = false;
     uint64_t total_candidates = 0;
     double workload_ms = 0;
     bool have_workload = false;
@@ -171,18 +173,25 @@ hegj_status ensure_workload(hegj_index *h) {
 // Default queries per work unit by dimensionality (tuned on MI355X).
 uint32_t default_qb(uint32_t dim) {
     if (dim <= 2) return 16;
-    if (dim <= 8) return 8;
+    if (dim <= 8) return 4;
     return 8;
 }
 
 struct Counters {
-    uint32_t queue_head;
+    uint32_t queue_heads[kMaxParts];
     uint32_t n_deferred;
+    uint32_t pad;
     unsigned long long cursor;
     unsigned long long hash;
     unsigned long long evals;
     unsigned long long pairs;
 };
+
+hegj_status ensure_morton(hegj_index *h) {
+    if (h->d_morton) return HEGJ_OK;
+    HIP_TRY(hipMalloc(&h->d_morton, sizeof(uint32_t) * (h->ix.C + 1)));
+    return morton_order(h->g, h->ix, h->d_morton, &h->morton_ok, h->stream);
+}
 
 }  // namespace
 
@@ -211,6 +220,15 @@ int hegj_device_count(void) {
     int c = 0;
     if (hipGetDeviceCount(&c) != hipSuccess) return 0;
     return c;
+}
+
+hegj_status hegj_stream_copy_bandwidth(uint64_t bytes, int device, double *gbps) {
+    if (!gbps || bytes < 4096) return fail(HEGJ_ERR_INVALID_ARG, "bytes must be >= 4096");
+    HEGJ_TRY(set_device(device));
+    double ms = 0;
+    HEGJ_TRY(stream_copy(bytes, 10, &ms));
+    *gbps = 2.0 * (double)(bytes / 16 * 16) / (ms * 1e-3) / 1e9;  // read + write
+    return HEGJ_OK;
 }
 
 void hegj_index_opts_default(hegj_index_opts *o, uint32_t dim) {
@@ -252,6 +270,7 @@ hegj_status hegj_index_free(hegj_index *h) {
     free_index(&h->ix);
     if (h->d_work) (void)hipFree(h->d_work);
     if (h->d_order) (void)hipFree(h->d_order);
+    if (h->d_morton) (void)hipFree(h->d_morton);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return HEGJ_OK;
@@ -344,7 +363,7 @@ hegj_status hegj_sort_by_workload(hegj_index *h, hegj_scheduling_cell *sorted, u
 
 void hegj_join_opts_default(hegj_join_opts *o) {
     std::memset(o, 0, sizeof(*o));
-    o->unit_order = HEGJ_ORDER_SPATIAL;
+    o->unit_order = HEGJ_ORDER_MORTON;
 }
 
 hegj_status hegj_selfjoin(hegj_index *h, const hegj_join_opts *opts_in, hegj_result **out) {
@@ -372,7 +391,15 @@ hegj_status hegj_selfjoin(hegj_index *h, const hegj_join_opts *opts_in, hegj_res
     Scratch sc;
     uint2 *units = nullptr;
     uint32_t U = 0;
-    const uint32_t *order = opts.unit_order == HEGJ_ORDER_WORKLOAD ? h->d_order : nullptr;
+    const uint32_t *order = nullptr;
+    if (opts.unit_order == HEGJ_ORDER_WORKLOAD) {
+        order = h->d_order;
+    } else if (opts.unit_order == HEGJ_ORDER_MORTON) {
+        HEGJ_TRY(ensure_morton(h));
+        if (h->morton_ok) order = h->d_morton;
+    } else if (opts.unit_order != HEGJ_ORDER_SPATIAL) {
+        return fail(HEGJ_ERR_INVALID_ARG, "unknown unit_order");
+    }
     HEGJ_TRY(build_units(h->g, h->ix, qb, order, &units, &U, s));
     sc.ptrs.push_back(units);
     r->info.n_units = U;
@@ -391,11 +418,15 @@ hegj_status hegj_selfjoin(hegj_index *h, const hegj_join_opts *opts_in, hegj_res
 
     uint32_t max_blocks = 0;
     HEGJ_TRY(join_occupancy(h->g.dim, qb, h->device, &max_blocks));
+    // per-wave hit-mask scratch for units whose masks exceed LDS
+    const uint32_t gmask_slots = 8192;
+    uint64_t *gmask;
+    HIP_TRY(sc.get(&gmask, (size_t)max_blocks * 4 * gmask_slots));
 
     JoinArgs a{};
     a.g = h->g;
     a.ix = h->ix;
-    a.queue_head = &d_ctr->queue_head;
+    a.queue_heads = d_ctr->queue_heads;
     a.n_deferred = &d_ctr->n_deferred;
     a.cursor = &d_ctr->cursor;
     a.hash = &d_ctr->hash;
@@ -404,9 +435,20 @@ hegj_status hegj_selfjoin(hegj_index *h, const hegj_join_opts *opts_in, hegj_res
     a.row_off = r->d_row_off;
     a.row_cnt = r->d_row_cnt;
     a.deferred = deferred;
+    a.gmask = gmask;
+    a.gmask_slots = gmask_slots;
+    a.part_by_xcc = opts.part_by_block ? 0u : 1u;
 
+    auto set_parts = [&](JoinArgs &x, uint32_t nu, uint32_t parts, const uint32_t *ends) {
+        x.n_parts = parts;
+        for (uint32_t p = 0; p < parts; p++) {
+            x.part_begin[p] = p == 0 ? 0u : ends[p - 1];
+            x.part_end[p] = ends[p];
+        }
+        (void)nu;
+    };
     auto grab_for = [&](uint32_t nu) {
-        // about 32 grabs per wave keeps the queue atomic off the critical path
+        // about 32 grabs per wave keeps the queue atomics off the critical path
         uint64_t waves = (uint64_t)max_blocks * 4;
         uint64_t g = nu / (waves * 32);
         return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, 64));
@@ -415,6 +457,11 @@ hegj_status hegj_selfjoin(hegj_index *h, const hegj_join_opts *opts_in, hegj_res
         uint64_t need = ((uint64_t)nu + grab * 4 - 1) / (grab * 4);
         return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(need, max_blocks));
     };
+    // unit ranges per XCD, split by estimated work
+    uint32_t parts = opts.xcd_parts ? std::min<uint32_t>(opts.xcd_parts, kMaxParts) : kMaxParts;
+    if (U < parts * 256) parts = 1;
+    uint32_t part_end[kMaxParts];
+    HEGJ_TRY(split_units(h->ix, units, U, qb, h->d_work, parts, part_end, s));
 
     // -- batch estimate: count-only pass over a strided unit sample ------------
     uint64_t capacity = opts.result_capacity;
@@ -431,6 +478,9 @@ hegj_status hegj_selfjoin(hegj_index *h, const hegj_join_opts *opts_in, hegj_res
         e.n_units = ns;
         e.units_per_grab = grab_for(ns);
         e.count_only = 1;
+        e.gmask_slots = 0;
+        uint32_t one_end = ns;
+        set_parts(e, ns, 1, &one_end);
         HEGJ_TRY(launch_join(e, qb, blocks_for(ns, e.units_per_grab), s));
         Counters c;
         HIP_TRY(hipMemcpyAsync(&c, d_ctr, sizeof(c), hipMemcpyDeviceToHost, s));
@@ -460,9 +510,17 @@ hegj_status hegj_selfjoin(hegj_index *h, const hegj_join_opts *opts_in, hegj_res
         r->bases.push_back(base);
         // reset per-batch counters (hash / evals / pairs accumulate)
         HIP_TRY(hipMemsetAsync(d_ctr, 0, offsetof(Counters, hash), s));
+        a.units_per_grab = grab_for(parts > 1 && cur_n == U ? cur_n / parts : cur_n);
+        if (opts.units_per_grab) a.units_per_grab = opts.units_per_grab;
         a.units = cur;
         a.n_units = cur_n;
         a.units_per_grab = grab_for(cur_n);
+        if (cur == units && cur_n == U) {
+            set_parts(a, cur_n, parts, part_end);
+        } else {
+            uint32_t one_end = cur_n;
+            set_parts(a, cur_n, 1, &one_end);
+        }
         a.out = buf;
         a.out_base = base;
         a.capacity = capacity;

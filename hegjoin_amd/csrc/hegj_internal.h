@@ -95,13 +95,20 @@ hegj_status radix_sort_u32(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt,
                            uint32_t *vals_alt, uint32_t n, int bits, hipStream_t s);
 
 // Arguments of the join kernel (hegj_join.hip).
+constexpr int kMaxParts = 8;   // one unit range (queue) per XCD
 struct JoinArgs {
     GridGeom g;
     DevIndex ix;
     const uint2 *units;           // (cell, first query offset in cell)
     uint32_t n_units;
     uint32_t units_per_grab;
-    uint32_t *queue_head;
+    uint32_t n_parts;             // unit ranges; a workgroup starts on its XCD's range
+    uint32_t part_by_xcc;         // 1: range = hardware XCC id, 0: blockIdx % n_parts
+    uint32_t part_begin[kMaxParts];
+    uint32_t part_end[kMaxParts];
+    uint32_t *queue_heads;        // [kMaxParts] units taken per range
+    uint64_t *gmask;              // per-wave hit-mask scratch (global)
+    uint32_t gmask_slots;         // masks per wave in gmask
     uint32_t *out;                // this batch's neighbour buffer
     uint64_t out_base;            // global offset of out[0]
     uint64_t capacity;            // entries in out
@@ -129,6 +136,24 @@ hegj_status launch_join(const JoinArgs &a, uint32_t qb, uint32_t blocks, hipStre
 // Workgroups of the persistent join kernel that fit on the device at once.
 hegj_status join_occupancy(uint32_t dim, uint32_t qb, int device, uint32_t *blocks);
 
+// ---- MFMA guard-band refine (hegj_mfma.hip) --------------------------------
+struct GbOutput {
+    uint32_t *buf = nullptr;          // neighbour ids, rows contiguous (allocated here)
+    uint64_t *row_off = nullptr;      // [n] (caller-allocated)
+    uint32_t *row_cnt = nullptr;      // [n] (caller-allocated, zeroed)
+    unsigned long long *hash = nullptr;  // device counter (caller-allocated, zeroed)
+    uint64_t pairs = 0, band = 0, evals = 0;
+    float kernel_ms = 0;
+};
+uint32_t gb_kblocks(uint32_t dim);        // K blocks of 32 for dim, 0 if unsupported
+uint32_t gb_unit_queries(uint32_t kb);    // queries per work unit of the MFMA kernel
+double gb_coefficient(uint32_t dim, uint32_t kp);
+hegj_status gb_prepare(const GridGeom &g, const DevIndex &ix, uint32_t kp, uint16_t **Av,
+                       uint16_t **Bv, float **w, hipStream_t s);
+hegj_status gb_join(const GridGeom &g, const DevIndex &ix, const uint2 *units, uint32_t n_units,
+                    const uint16_t *Av, const uint16_t *Bv, const float *w, uint32_t kp,
+                    int device, hipStream_t s, GbOutput *out);
+
 // ---- utilities (hegj_util.hip) -------------------------------------------
 hegj_status workload_order(const DevIndex &ix, const uint32_t *work, uint32_t *order,
                            uint64_t *total_candidates, hipStream_t s);
@@ -136,12 +161,23 @@ hegj_status origin_point_index(const DevIndex &ix, const uint32_t *order, uint32
                                hipStream_t s);
 hegj_status stride_units(const uint2 *in, uint32_t n_out, uint32_t stride, uint2 *out,
                          hipStream_t s);
+// Morton (bit-interleaved cell coordinates) order of the cells; false in
+// *ok when the coordinates need more than 64 bits.
+hegj_status morton_order(const GridGeom &g, const DevIndex &ix, uint32_t *order, bool *ok,
+                         hipStream_t s);
+// Splits the unit list into n_parts ranges of about equal estimated work
+// (queries x stencil candidates of the unit's cell).
+hegj_status split_units(const DevIndex &ix, const uint2 *units, uint32_t n_units, uint32_t qb,
+                        const uint32_t *work, uint32_t n_parts, uint32_t *part_end,
+                        hipStream_t s);
 hegj_status gather_units(const uint2 *in, const uint32_t *idx, uint32_t n, uint2 *out,
                          hipStream_t s);
 hegj_status csr_offsets(const uint32_t *row_cnt, uint32_t n, uint64_t *csr_off, hipStream_t s);
 hegj_status gather_rows(const std::vector<uint32_t *> &batches, const std::vector<uint64_t> &bases,
                         const uint64_t *row_off, const uint32_t *row_cnt, const uint64_t *csr_off,
                         uint32_t n, uint32_t *out, hipStream_t s);
+
+hegj_status stream_copy(uint64_t bytes, int reps, double *ms_per_copy);
 
 // ---- device helpers --------------------------------------------------------
 __device__ __forceinline__ uint64_t mix64(uint64_t x) {
@@ -159,6 +195,11 @@ __device__ __forceinline__ uint32_t hash_slot(uint64_t key) {
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Id of the XCD (accelerator complex die) the wave runs on (hwreg XCC_ID).
+__device__ __forceinline__ uint32_t xcc_id() {
+    return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15u;
 }
 
 __device__ __forceinline__ uint32_t wave_bcast_u32(uint32_t v) {
@@ -245,8 +286,16 @@ __device__ __forceinline__ void stencil_row(const uint32_t *c, const uint32_t *l
     if (row == ~0u) return;
     uint32_t rs = ix.row_start[row], re = ix.row_start[row + 1];
     uint64_t base = rkey * g.nCells[0];
-    uint32_t a = lower_bound_u64(ix.cell_key, rs, re, base + lo[0]);
-    uint32_t b = lower_bound_u64(ix.cell_key, a, re, base + hi[0] + 1);
+    uint32_t a, b;
+    if (re - rs <= 8) {  // short rows (the common case at k >= 4): scan
+        a = rs;
+        while (a < re && ix.cell_key[a] < base + lo[0]) a++;
+        b = a;
+        while (b < re && ix.cell_key[b] <= base + hi[0]) b++;
+    } else {
+        a = lower_bound_u64(ix.cell_key, rs, re, base + lo[0]);
+        b = lower_bound_u64(ix.cell_key, a, re, base + hi[0] + 1);
+    }
     *cfirst = a;
     *clast = b;
 }

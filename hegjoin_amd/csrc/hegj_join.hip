@@ -35,7 +35,6 @@ namespace hegj {
 
 namespace {
 
-constexpr int kMaxChunks = 64;   // hit masks kept in LDS per query
 constexpr int kWavesPerBlock = 4;
 
 // Compacted stencil-row list of one unit in LDS.
@@ -93,7 +92,8 @@ __device__ uint32_t build_rows(const JoinArgs &a, uint32_t cell, RowList &rl, ui
     return total;
 }
 
-// Candidate position of flattened candidate index t (t < M).
+// Candidate position of flattened candidate index t (t < M), by binary search
+// over the slot prefix.
 __device__ __forceinline__ uint32_t cand_pos(const RowList &rl, uint32_t nslots, uint32_t t) {
     uint32_t lo = 0, hi = nslots;  // last slot with off <= t
     while (hi - lo > 1) {
@@ -106,14 +106,52 @@ __device__ __forceinline__ uint32_t cand_pos(const RowList &rl, uint32_t nslots,
     return rl.start[lo] + (t - rl.off[lo]);
 }
 
-// acc[qi] for one candidate chunk against the unit's queries.
+// Same, advancing a per-lane slot cursor (t never decreases for a lane).
+__device__ __forceinline__ uint32_t cand_pos_adv(const RowList &rl, uint32_t nslots, uint32_t t,
+                                                 uint32_t &slot) {
+    while (slot + 1 < nslots && rl.off[slot + 1] <= t) slot++;
+    return rl.start[slot] + (t - rl.off[slot]);
+}
+
+// Candidate coordinates: D doubles from the cell-ordered SoA copy.
+template <int D>
+__device__ __forceinline__ void load_cand(const JoinArgs &a, uint32_t pos, bool valid, double *x) {
+    const uint64_t n = a.g.n;
+#pragma unroll
+    for (int l = 0; l < D; l++) x[l] = valid ? a.ix.soa[(uint64_t)l * n + pos] : 0.0;
+}
+
+// acc[qi] for one candidate (coordinates in x) against the unit's queries:
+// acc = fma(c - q, c - q, acc) in dimension order (kernel.cu:350-353 with
+// FMA contraction).  Query coordinates are wave-uniform loads.
 template <int D, int QB>
-__device__ __forceinline__ void chunk_dist(const JoinArgs &a, uint32_t pos, bool valid,
-                                           uint32_t qpos0, uint32_t nq, double *acc) {
+__device__ __forceinline__ void dist_regs(const JoinArgs &a, const double *x, uint32_t qpos0,
+                                          uint32_t nq, double *acc) {
+    const uint64_t n = a.g.n;
+    const double *__restrict__ soa = a.ix.soa;
+#pragma unroll
+    for (int qi = 0; qi < QB; qi++) {
+        acc[qi] = 0.0;
+        if ((uint32_t)qi < nq) {
+            const uint64_t qp = qpos0 + qi;
+#pragma unroll
+            for (int l = 0; l < D; l++) {
+                double dd = x[l] - soa[(uint64_t)l * n + qp];
+                acc[qi] = __builtin_fma(dd, dd, acc[qi]);
+            }
+        }
+    }
+}
+
+// Generic dimension count (runtime or large D): candidate coordinates are
+// streamed in tiles of 16 dims, acc keeps the dimension order.
+template <int D, int QB>
+__device__ __forceinline__ void dist_stream(const JoinArgs &a, uint32_t pos, bool valid,
+                                            uint32_t qpos0, uint32_t nq, double *acc) {
     const uint32_t dim = D > 0 ? (uint32_t)D : a.g.dim;
     const uint64_t n = a.g.n;
     const double *__restrict__ soa = a.ix.soa;
-    constexpr int DT = (D > 0 && D <= 32) ? D : 16;
+    constexpr int DT = 16;
 #pragma unroll
     for (int qi = 0; qi < QB; qi++) acc[qi] = 0.0;
     for (uint32_t d0 = 0; d0 < dim; d0 += DT) {
@@ -131,8 +169,7 @@ __device__ __forceinline__ void chunk_dist(const JoinArgs &a, uint32_t pos, bool
                 for (int i = 0; i < DT; i++) {
                     uint32_t l = d0 + i;
                     if (l < dim) {
-                        double q = soa[(uint64_t)l * n + qp];
-                        double dd = x[i] - q;
+                        double dd = x[i] - soa[(uint64_t)l * n + qp];
                         acc[qi] = __builtin_fma(dd, dd, acc[qi]);
                     }
                 }
@@ -141,23 +178,55 @@ __device__ __forceinline__ void chunk_dist(const JoinArgs &a, uint32_t pos, bool
     }
 }
 
+// Hit masks of one unit: mask[qi * nchunks + ch], one 64-bit ballot per
+// (query, candidate chunk).  Kept in LDS when they fit, else in a per-wave
+// global scratch slice (written and read back by the same wave, so L2
+// resident), else (very large units) recomputed in the write phase.
+constexpr int kMaskSlots = 512;
+struct HitStore {
+    uint64_t w[kMaskSlots];
+};
+
+constexpr int kRegDims = 32;  // dims kept in registers (with prefetch)
+
+// Takes the next grab of units: from the workgroup's own range first (its
+// XCD's share of the space, for L2 locality), then from the others.
+__device__ __forceinline__ bool next_grab(const JoinArgs &a, uint32_t &part, uint32_t &tries,
+                                          uint32_t &grab, uint32_t &gend) {
+    const int lane = threadIdx.x & 63;
+    while (tries < a.n_parts) {
+        uint32_t g = 0;
+        if (lane == 0) g = atomicAdd(&a.queue_heads[part], a.units_per_grab);
+        g = wave_bcast_u32(g) + a.part_begin[part];
+        if (g < a.part_end[part]) {
+            grab = g;
+            gend = min(g + a.units_per_grab, a.part_end[part]);
+            return true;
+        }
+        part = part + 1 == a.n_parts ? 0 : part + 1;
+        tries++;
+    }
+    return false;
+}
+
 template <int D, int QB>
 __global__ __launch_bounds__(64 * kWavesPerBlock) void k_join(JoinArgs a) {
     __shared__ RowList rls[kWavesPerBlock];
-    __shared__ uint64_t masks[kWavesPerBlock][QB][kMaxChunks];
+    __shared__ HitStore hs_all[kWavesPerBlock];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     RowList &rl = rls[wid];
+    uint64_t *const hs_lds = hs_all[wid].w;
+    uint64_t *const hs_glb = a.gmask + (uint64_t)(blockIdx.x * kWavesPerBlock + wid) * a.gmask_slots;
     const double T = a.g.thresh;
+    constexpr bool kRegs = (D > 0 && D <= kRegDims);
+    constexpr int DR = kRegs ? D : 1;
     uint64_t my_hash = 0, my_evals = 0, my_pairs = 0;
+    uint32_t part = (a.part_by_xcc ? xcc_id() : blockIdx.x) % a.n_parts;
+    uint32_t tries = 0, grab = 0, gend = 0;
 
-    while (true) {
-        uint32_t grab = 0;
-        if (lane == 0) grab = atomicAdd(a.queue_head, a.units_per_grab);
-        grab = wave_bcast_u32(grab);
-        if (grab >= a.n_units) break;
-        uint32_t gend = min(grab + a.units_per_grab, a.n_units);
+    while (next_grab(a, part, tries, grab, gend)) {
         for (uint32_t ui = grab; ui < gend; ui++) {
-            uint2 unit = a.units[ui];
+            const uint2 unit = a.units[ui];
             const uint32_t cell = wave_bcast_u32(unit.x), qoff = wave_bcast_u32(unit.y);
             const uint32_t cstart = a.ix.cell_start[cell];
             const uint32_t csize = a.ix.cell_start[cell + 1] - cstart;
@@ -167,23 +236,61 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_join(JoinArgs a) {
             bool live;
             const uint32_t M = build_rows(a, cell, rl, &nslots, &live);
             wave_lds_sync();
+            const uint32_t nchunks = (M + 63) / 64;
+            const uint32_t nmask = nq * nchunks;
+            const bool keep = nmask <= a.gmask_slots || nmask <= (uint32_t)kMaskSlots;
+            uint64_t *const hs = nmask <= (uint32_t)kMaskSlots ? hs_lds : hs_glb;
             uint32_t cnt[QB];
 #pragma unroll
             for (int qi = 0; qi < QB; qi++) cnt[qi] = 0;
-            const uint32_t nchunks = (M + 63) / 64;
-            for (uint32_t ch = 0; ch < nchunks; ch++) {
-                uint32_t t = ch * 64 + lane;
-                bool valid = t < M;
-                uint32_t pos = valid ? cand_pos(rl, nslots, t) : 0u;
-                double acc[QB];
-                chunk_dist<D, QB>(a, pos, valid, qpos0, nq, acc);
+
+            uint32_t slot = 0;
+            // Candidate coordinates are prefetched kDepth chunks ahead in a
+            // register ring (the loop is unrolled by kDepth so ring indices
+            // are static): several chunks of loads stay in flight per wave.
+            constexpr int kDepth = kRegs ? (D <= 8 ? 4 : 2) : 1;
+            double ring[kDepth][DR];
+            if (kRegs) {
 #pragma unroll
-                for (int qi = 0; qi < QB; qi++) {
-                    if ((uint32_t)qi < nq) {
-                        uint64_t m = __ballot(valid && acc[qi] <= T);
-                        cnt[qi] += (uint32_t)__popcll(m);
-                        if (ch < kMaxChunks && lane == 0) masks[wid][qi][ch] = m;
+                for (int p = 0; p < kDepth; p++) {
+                    const uint32_t t = p * 64 + lane;
+                    const bool v = t < M;
+                    load_cand<DR>(a, v ? cand_pos_adv(rl, nslots, t, slot) : 0u, v, ring[p]);
+                }
+            }
+            for (uint32_t ch0 = 0; ch0 < nchunks; ch0 += kDepth) {
+#pragma unroll
+                for (int p = 0; p < kDepth; p++) {
+                    const uint32_t ch = ch0 + p;
+                    if (ch >= nchunks) break;
+                    double acc[QB];
+                    const bool valid = ch * 64 + lane < M;
+                    if (kRegs) {
+                        double x[DR];
+#pragma unroll
+                        for (int l = 0; l < DR; l++) x[l] = ring[p][l];
+                        if (ch + kDepth < nchunks) {
+                            const uint32_t tn = (ch + kDepth) * 64 + lane;
+                            const bool vn = tn < M;
+                            load_cand<DR>(a, vn ? cand_pos_adv(rl, nslots, tn, slot) : 0u, vn,
+                                          ring[p]);
+                        }
+                        dist_regs<DR, QB>(a, x, qpos0, nq, acc);
+                    } else {
+                        const uint32_t t = ch * 64 + lane;
+                        const uint32_t pos = valid ? cand_pos_adv(rl, nslots, t, slot) : 0u;
+                        dist_stream<D, QB>(a, pos, valid, qpos0, nq, acc);
                     }
+                    uint64_t mine = 0;  // lane qi keeps query qi's mask
+#pragma unroll
+                    for (int qi = 0; qi < QB; qi++) {
+                        if ((uint32_t)qi < nq) {
+                            const uint64_t m = __ballot(valid && acc[qi] <= T);
+                            cnt[qi] += (uint32_t)__popcll(m);
+                            if (lane == qi) mine = m;
+                        }
+                    }
+                    if (keep && (uint32_t)lane < nq) hs[lane * nchunks + ch] = mine;
                 }
             }
             my_evals += (uint64_t)M * nq;
@@ -208,35 +315,43 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_join(JoinArgs a) {
             }
             my_pairs += total;
             wave_lds_sync();
-            // write phase
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            // row table entries of the unit's queries
+            if ((uint32_t)lane < nq) {
+                uint32_t myexcl = 0, mycnt = 0;
+#pragma unroll
+                for (int qi = 0; qi < QB; qi++)
+                    if (qi == lane) {
+                        myexcl = excl[qi];
+                        mycnt = cnt[qi];
+                    }
+                const uint32_t qid = a.ix.ila[qpos0 + lane];
+                a.row_off[qid] = a.out_base + base + myexcl;
+                a.row_cnt[qid] = mycnt;
+            }
+            if (total == 0) continue;
 #pragma unroll
             for (int qi = 0; qi < QB; qi++) {
-                if ((uint32_t)qi >= nq) continue;
+                if ((uint32_t)qi >= nq || cnt[qi] == 0) continue;
                 const uint32_t qid = a.ix.ila[qpos0 + qi];
-                if (lane == 0) {
-                    a.row_off[qid] = a.out_base + base + excl[qi];
-                    a.row_cnt[qid] = cnt[qi];
-                }
-                if (cnt[qi] == 0) continue;
                 uint64_t o = base + excl[qi];
+                uint32_t slot2 = 0;
                 for (uint32_t ch = 0; ch < nchunks; ch++) {
+                    const uint32_t t = ch * 64 + lane;
                     uint64_t m;
-                    if (ch < kMaxChunks) {
-                        m = masks[wid][qi][ch];
+                    if (keep) {
+                        m = hs[qi * nchunks + ch];
                     } else {
-                        // masks beyond the LDS budget are recomputed
-                        uint32_t t = ch * 64 + lane;
-                        bool valid = t < M;
-                        uint32_t pos = valid ? cand_pos(rl, nslots, t) : 0u;
+                        const bool valid = t < M;
+                        const uint32_t pos = valid ? cand_pos(rl, nslots, t) : 0u;
                         double acc[1];
-                        chunk_dist<D, 1>(a, pos, valid, qpos0 + qi, 1, acc);
+                        dist_stream<D, 1>(a, pos, valid, qpos0 + qi, 1, acc);
                         m = __ballot(valid && acc[0] <= T);
                         my_evals += (uint64_t)min(64u, M - ch * 64);
                     }
                     if (m == 0) continue;
                     if ((m >> lane) & 1ull) {
-                        uint32_t pos = cand_pos(rl, nslots, ch * 64 + lane);
-                        uint32_t id = a.ix.ila[pos];
+                        const uint32_t id = a.ix.ila[cand_pos_adv(rl, nslots, t, slot2)];
                         a.out[o + lanes_below(m)] = id;
                         my_hash += mix64(((uint64_t)qid << 32) | id);
                     }
@@ -246,9 +361,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_join(JoinArgs a) {
         }
     }
     // one atomic per wave for the counters
-    for (int o = 32; o > 0; o >>= 1) {
-        my_hash += __shfl_xor(my_hash, o, 64);
-    }
+    for (int o = 32; o > 0; o >>= 1) my_hash += __shfl_xor(my_hash, o, 64);
     if (lane == 0) {
         if (my_hash) atomicAdd(a.hash, (unsigned long long)my_hash);
         if (my_evals) atomicAdd(a.evals, (unsigned long long)my_evals);

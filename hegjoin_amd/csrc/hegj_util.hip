@@ -2,6 +2,8 @@
 // sortByWorkLoad (SortByWorkload.cu:96-217), unit list gathers, and the
 // neighbour-table to CSR gather (the device form of
 // constructNeighborTableKeyValueWithPtrs, GPU.cu:1731-1776).
+#include <algorithm>
+
 #include "hegj_internal.h"
 
 namespace hegj {
@@ -182,6 +184,149 @@ hegj_status gather_rows(const std::vector<uint32_t *> &batches, const std::vecto
     bt.base[batches.size()] = ~0ull;
     k_gather_rows<<<ceil_div((uint64_t)n * 64, kT), kT, 0, s>>>(bt, row_off, row_cnt, csr_off, n, out);
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    return HEGJ_OK;
+}
+
+}  // namespace hegj
+
+namespace hegj {
+namespace {
+// Streaming copy (16-byte vectors, grid-stride): the reference point for HBM
+// counters and the attainable bandwidth of the device.
+__global__ __launch_bounds__(256) void k_stream_copy(const uint4 *__restrict__ src,
+                                                     uint4 *__restrict__ dst, uint64_t n16) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n16;
+         i += (uint64_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+}  // namespace
+
+hegj_status stream_copy(uint64_t bytes, int reps, double *ms_per_copy) {
+    uint64_t n16 = bytes / 16;
+    Scratch sc;
+    uint4 *a, *b;
+    HIP_TRY(sc.get(&a, n16));
+    HIP_TRY(sc.get(&b, n16));
+    HIP_TRY(hipMemset(a, 1, n16 * 16));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    k_stream_copy<<<8192, 256>>>(a, b, n16);  // warm
+    HIP_TRY(hipEventRecord(e0, nullptr));
+    for (int r = 0; r < reps; r++) k_stream_copy<<<8192, 256>>>(a, b, n16);
+    HIP_TRY(hipEventRecord(e1, nullptr));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_per_copy = ms / reps;
+    return HEGJ_OK;
+}
+
+}  // namespace hegj
+
+namespace hegj {
+namespace {
+
+// Morton key of each cell: bits of the k cell coordinates interleaved
+// (coordinate j's bit b goes to position b*k + j).
+__global__ __launch_bounds__(256) void k_morton_keys(GridGeom g, DevIndex ix, uint32_t bits,
+                                                     uint64_t *__restrict__ keys,
+                                                     uint32_t *__restrict__ vals) {
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= ix.C) return;
+    uint32_t c[kMaxK];
+    decode_cell(ix.cell_key[i], g, c);
+    uint64_t key = 0;
+    for (uint32_t b = 0; b < bits; b++)
+        for (uint32_t j = 0; j < g.k; j++) key |= (uint64_t)((c[j] >> b) & 1u) << (b * g.k + j);
+    keys[i] = key;
+    vals[i] = i;
+}
+
+// Estimated work of each unit: queries x candidates of its cell.
+__global__ __launch_bounds__(256) void k_unit_work(const uint2 *__restrict__ units, uint32_t n,
+                                                   const uint32_t *__restrict__ cell_start,
+                                                   const uint32_t *__restrict__ work, uint32_t qb,
+                                                   uint64_t *__restrict__ w) {
+    uint32_t u = blockIdx.x * 256 + threadIdx.x;
+    if (u >= n) return;
+    uint2 un = units[u];
+    uint32_t sz = cell_start[un.x + 1] - cell_start[un.x];
+    uint32_t nq = min(qb, sz - un.y);
+    w[u] = (uint64_t)nq * (work[un.x] + 64);  // + per-unit overhead
+}
+
+// part_end[p] = first unit whose exclusive work prefix reaches (p+1)/n of
+// the total (the last part ends at n).
+__global__ void k_find_splits(const uint64_t *__restrict__ prefix, uint32_t n,
+                              const uint64_t *__restrict__ total, uint32_t parts,
+                              uint32_t *__restrict__ part_end) {
+    uint32_t p = threadIdx.x;
+    if (p >= parts) return;
+    if (p == parts - 1) {
+        part_end[p] = n;
+        return;
+    }
+    uint64_t target = (*total) / parts * (p + 1);
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (prefix[mid] < target)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    part_end[p] = lo;
+}
+
+}  // namespace
+
+hegj_status morton_order(const GridGeom &g, const DevIndex &ix, uint32_t *order, bool *ok,
+                         hipStream_t s) {
+    uint32_t maxc = 1;
+    for (uint32_t j = 0; j < g.k; j++) maxc = std::max(maxc, g.nCells[j]);
+    uint32_t bits = 0;
+    while (bits < 32 && ((maxc - 1) >> bits) != 0) bits++;
+    if (bits * g.k > 64 || ix.C == 0) {
+        *ok = false;
+        return HEGJ_OK;
+    }
+    *ok = true;
+    Scratch sc;
+    uint64_t *keys, *keys_alt;
+    uint32_t *vals_alt;
+    HIP_TRY(sc.get(&keys, ix.C));
+    HIP_TRY(sc.get(&keys_alt, ix.C));
+    HIP_TRY(sc.get(&vals_alt, ix.C));
+    k_morton_keys<<<ceil_div(ix.C, 256), 256, 0, s>>>(g, ix, bits, keys, order);
+    HIP_TRY(hipGetLastError());
+    HEGJ_TRY(radix_sort_u64(keys, order, keys_alt, vals_alt, ix.C, (int)(bits * g.k), s));
+    return HEGJ_OK;
+}
+
+hegj_status split_units(const DevIndex &ix, const uint2 *units, uint32_t n_units, uint32_t qb,
+                        const uint32_t *work, uint32_t n_parts, uint32_t *part_end,
+                        hipStream_t s) {
+    if (n_parts <= 1 || n_units == 0) {
+        part_end[0] = n_units;
+        return HEGJ_OK;
+    }
+    Scratch sc;
+    uint64_t *w, *prefix, *total;
+    uint32_t *d_end;
+    HIP_TRY(sc.get(&w, n_units));
+    HIP_TRY(sc.get(&prefix, n_units));
+    HIP_TRY(sc.get(&total, 1));
+    HIP_TRY(sc.get(&d_end, kMaxParts));
+    k_unit_work<<<ceil_div(n_units, 256), 256, 0, s>>>(units, n_units, ix.cell_start, work, qb, w);
+    HIP_TRY(hipGetLastError());
+    HEGJ_TRY(scan_u64(w, prefix, n_units, total, s));
+    k_find_splits<<<1, 64, 0, s>>>(prefix, n_units, total, n_parts, d_end);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(part_end, d_end, sizeof(uint32_t) * n_parts, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     return HEGJ_OK;
 }

@@ -73,6 +73,11 @@ const char *hegj_last_error(void);
 const char *hegj_version(void);
 /* Number of visible HIP devices (0 when none). */
 int hegj_device_count(void);
+/* Attainable HBM bandwidth: a streaming 16-byte-vector copy of `bytes`
+ * between two device buffers, (read + write bytes) / time, in GB/s.  Not part
+ * of the reference; it is the measured roof the join's HBM figures are
+ * compared with. */
+hegj_status hegj_stream_copy_bandwidth(uint64_t bytes, int device, double *gbps);
 
 /* ------------------------------------------------------------------------ */
 /* Datasets.                                                                */
@@ -155,15 +160,24 @@ typedef struct hegj_result hegj_result;
 
 typedef enum {
     HEGJ_ORDER_SPATIAL = 0,   /* work units in cell (linear id) order */
-    HEGJ_ORDER_WORKLOAD = 1   /* work units by descending cell workload */
+    HEGJ_ORDER_WORKLOAD = 1,  /* work units by descending cell workload */
+    HEGJ_ORDER_MORTON = 2     /* work units in Morton order of cell coordinates */
 } hegj_unit_order;
+
+typedef enum {
+    HEGJ_REFINE_AUTO = 0,     /* MFMA guard band when dim >= 16, else FP64 */
+    HEGJ_REFINE_FP64 = 1,     /* exact FP64 chain for every candidate */
+    HEGJ_REFINE_MFMA = 2      /* split-BF16 MFMA tiles + guard band + FP64 band recheck */
+} hegj_refine;
 
 typedef struct {
     uint64_t result_capacity; /* pairs per device result batch; 0 = estimate */
-    uint32_t unit_order;      /* hegj_unit_order */
-    uint32_t queries_per_unit;/* 0 = tuned default */
-    uint32_t block_size;      /* threads per workgroup, 0 = tuned default */
-    uint32_t reserved[3];
+    uint32_t unit_order;      /* hegj_unit_order (default: Morton) */
+    uint32_t queries_per_unit;/* FP64 kernel: 4, 8 or 16; 0 = tuned default */
+    uint32_t refine;          /* hegj_refine */
+    uint32_t xcd_parts;       /* unit ranges, one per XCD: 0 = auto (8), 1 = one queue */
+    uint32_t units_per_grab;  /* units a wave takes per queue atomic; 0 = auto */
+    uint32_t part_by_block;   /* 0: range by hardware XCD id, 1: by workgroup index */
 } hegj_join_opts;
 
 void hegj_join_opts_default(hegj_join_opts *opts);

@@ -87,11 +87,12 @@ def test_join_without_cell0_wrap(H, path):
 
 
 @pytest.mark.parametrize("qb", [4, 8, 16])
-@pytest.mark.parametrize("order", ["spatial", "workload"])
-def test_join_variants_identical(H, qb, order):
+@pytest.mark.parametrize("order", ["spatial", "workload", "morton"])
+@pytest.mark.parametrize("parts", [1, 8])
+def test_join_variants_identical(H, qb, order, parts):
     g = load_golden(golden_files("cell0_d2")[0])
     off, nb, info = gpu_join(H, g["pts"], float(g["eps"]), int(g["k"]), queries_per_unit=qb,
-                             unit_order=order)
+                             unit_order=order, xcd_parts=parts)
     assert np.array_equal(off, g["offsets"]) and np.array_equal(nb, g["nbrs"])
     assert info["pair_hash"] == int(g["hash"])
 
@@ -120,6 +121,20 @@ def test_join_random_vs_oracle(H, oracle, dim, k, n, eps, seed):
     eoff, enb = oracle.join(pts, k, eps)
     assert np.array_equal(off, eoff) and np.array_equal(nb, enb)
     assert info["pair_hash"] == oracle.csr_hash(eoff, enb)
+
+
+@pytest.mark.parametrize("n,eps", [(40000, 0.004), (60000, 0.005)])
+def test_large_units_use_global_masks(H, oracle, n, eps):
+    """Slabs (k = 1) holding thousands of points: with 16 queries per unit
+    the (query, chunk) masks exceed LDS (first case: per-wave global
+    scratch) or even that scratch (second case, more than 8192 masks:
+    recomputed in the write phase)."""
+    rng = np.random.default_rng(3)
+    pts = rng.random((n, 3))
+    pts[:, 0] *= 0.02
+    off, nb, info = gpu_join(H, pts, eps, 1, queries_per_unit=16)
+    eoff, enb = oracle.join(pts, 1, eps)
+    assert np.array_equal(off, eoff) and np.array_equal(nb, enb)
 
 
 def test_duplicate_points_and_tiny_inputs(H, oracle):
