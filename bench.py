@@ -43,7 +43,11 @@ CONFIGS = {
     "90d": dict(dim=90, n=1_000_000, k=6, eps=0.19773135137186568, target=16,
                 name="90-D 1M expo FP64 k=6 eps calibrated for 16 neighbours"),
 }
+SERVE = dict(dim=6, n=10_000_000, k=6, nq_lo=1_000, nq_hi=500_000, eps_lo=0.002, eps_hi=0.02,
+             name="serving: 6-D 10M expo FP64 resident index; Poisson arrivals; request queries "
+                  "log-uniform 1k-500k, eps log-uniform [0.002, 0.02]")
 PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
+PEAK_BF16_TFLOPS = 2500.0 # MI355X dense bf16 MFMA
 PEAK_HBM_GBS = 8000.0     # MI355X HBM3E
 
 
@@ -87,12 +91,108 @@ def cpu_baseline(pts, eps, dim, seconds_budget=20.0):
             "kind": "port", "sample": f"Super-EGO restatement, {len(qids)} queries"}
 
 
+def serve_levels(eps_lo, eps_hi):
+    """Grid levels for the serving index: cell sides from just above eps_hi
+    down by factors of sqrt(2) while they still exceed eps_lo / sqrt(2)."""
+    top = eps_hi * (1 + 1e-6)
+    sides = [top]
+    while sides[-1] / np.sqrt(2) >= eps_lo * 0.999:
+        sides.append(sides[-1] / np.sqrt(2))
+    return sorted(sides)[:8]
+
+
+def serve_main(a):
+    """Serving: requests arrive as a Poisson stream at --qps against the
+    resident index; latency = submit -> result ready in device memory."""
+    import threading
+    import hegjoin_amd as H
+    c = SERVE
+    t0 = time.time()
+    pts = H.generate_expo(c["n"], c["dim"], seed=0)
+    sides = serve_levels(c["eps_lo"], c["eps_hi"])
+    srv = H.Server(pts, sides, indexed_dims=c["k"], slots=a.slots,
+                   result_bytes=int(a.pool_gb * (1 << 30)), max_queries=c["nq_hi"])
+    del pts
+    print(f"# index {c['n']}x{c['dim']} at {len(sides)} levels in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    rng = np.random.default_rng(a.serve_seed)
+    total = a.warmup + a.requests
+    nqs = np.exp(rng.uniform(np.log(c["nq_lo"]), np.log(c["nq_hi"]), total)).astype(np.int64)
+    epss = np.exp(rng.uniform(np.log(c["eps_lo"]), np.log(c["eps_hi"]), total))
+    gaps = rng.exponential(1.0 / a.qps, total)
+    queries = [H.generate_expo(int(nqs[i]), c["dim"], seed=100_000 + i) for i in range(total)]
+
+    def run(lo, hi, timed):
+        tickets = [None] * (hi - lo)
+        ready = [threading.Event() for _ in range(hi - lo)]
+        start = time.perf_counter()
+        arrivals = start + np.cumsum(gaps[lo:hi]) - gaps[lo]
+
+        def submitter():
+            for j in range(hi - lo):
+                d = arrivals[j] - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                tickets[j] = srv.submit(queries[lo + j], epss[lo + j])
+                ready[j].set()
+        th = threading.Thread(target=submitter)
+        th.start()
+        infos = []
+        for j in range(hi - lo):
+            ready[j].wait()
+            try:
+                info = srv.wait(tickets[j])
+                info["ok"] = True
+            except H.HegjError as e:
+                info = {"ok": False, "error": str(e), "n_queries": int(nqs[lo + j])}
+            info["eps"] = float(epss[lo + j])
+            srv.release(tickets[j])
+            infos.append(info)
+            if timed:
+                print(f"# request {j}: nq {info['n_queries']} eps {info['eps']:.4f} "
+                      + (f"pairs {info['pairs']} latency {info['latency_ms']:.1f} ms" if info["ok"]
+                         else "rejected: " + info["error"]), file=sys.stderr, flush=True)
+        th.join()
+        return infos, time.perf_counter() - start
+
+    if a.warmup:
+        run(0, a.warmup, False)
+    srv.stats(reset=True)
+    infos, wall = run(a.warmup, total, True)
+    st = srv.stats()
+    srv.close()
+    ok = [i for i in infos if i["ok"]]
+    pairs = sum(i["pairs"] for i in ok)
+    evals = sum(i["evals"] for i in ok)
+    out = {
+        "metric": "serving p50/p99 latency at fixed QPS, result pairs/s (1 GPU)",
+        "value": pairs / wall, "unit": "pairs/s",
+        "n_gpus": 1, "steps": a.requests, "warmup": a.warmup,
+        "ms_per_step": wall * 1000 / max(a.requests, 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (reference exponential generator)",
+        "config": {"workload": c["name"], "n": c["n"], "dim": c["dim"], "qps": a.qps,
+                   "requests": a.requests, "slots": a.slots, "pool_gb": a.pool_gb,
+                   "levels": [round(x, 6) for x in sides], "seed": a.serve_seed},
+        "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "max_ms": st["max_ms"],
+        "mean_ms": st["mean_ms"], "completed": len(ok), "rejected": len(infos) - len(ok),
+        "distance_evals_per_s": evals / wall, "pool_peak_bytes": st["pool_peak"],
+        "wall_s": wall,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="6d", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="6d", choices=sorted(CONFIGS) + ["serve"])
+    ap.add_argument("--qps", type=float, default=0.5, help="serving: request arrival rate")
+    ap.add_argument("--requests", type=int, default=20, help="serving: timed requests")
+    ap.add_argument("--slots", type=int, default=4, help="serving: requests on the GPU at once")
+    ap.add_argument("--pool-gb", type=float, default=96.0, help="serving: result-buffer pool")
+    ap.add_argument("--serve-seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--calibrate", action="store_true")
@@ -102,12 +202,18 @@ def main():
     ap.add_argument("--parts-by-block", type=int, default=0)
     ap.add_argument("--grab", type=int, default=0)
     ap.add_argument("--refine", default="auto", choices=["auto", "fp64", "mfma"])
+    ap.add_argument("--n", type=int, default=0, help="override the point count (experiments)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
                           "n_gpus": a.gpus}))
         sys.exit(2)
-    cfg = CONFIGS[a.config]
+    if a.config == "serve":
+        return serve_main(a)
+    cfg = dict(CONFIGS[a.config])
+    if a.n:
+        cfg["n"] = a.n
+        cfg["name"] += f" (n overridden to {a.n})"
     if a.calibrate:
         for key, c in CONFIGS.items():
             if c["target"]:
@@ -148,7 +254,18 @@ def main():
     # dominant kernel: the join; one launch per step when one batch suffices
     launches = sum(i["join_launches"] for i in infos)
     kernel_ms = sum(i["join_ms"] for i in infos) / launches
-    flops_alg = 3.0 * dim * total_cand / (launches / a.steps)  # sub + fma per dim per eval
+    refine_used = infos[-1]["refine_used"]
+    if refine_used == "mfma":
+        # split-BF16 Gram tile: K = 3 (dim + 2) products per candidate pair, 2 flops each
+        flops_alg = 2.0 * 3 * (dim + 2) * total_cand / (launches / a.steps)
+        peak_tf, roof_note = PEAK_BF16_TFLOPS, (
+            "MFMA bf16 dense: 2 * 3(dim+2) flops per reference candidate pair (split-BF16 "
+            "hi/lo Gram product), kernel k_gb_join")
+    else:
+        flops_alg = 3.0 * dim * total_cand / (launches / a.steps)  # sub + fma per dim per eval
+        peak_tf, roof_note = PEAK_FP64_TFLOPS, (
+            "FP64 compute bound: 3*dim flops (sub + fma per dim) per reference candidate; "
+            "FP64 vector and matrix peaks are equal, kernel k_join")
     achieved_tf = flops_alg / (kernel_ms * 1e-3) / 1e12
     bytes_alg = 8.0 * n * dim + 4.0 * n + 24.0 * iinfo["nNonEmptyCells"] + 4.0 * pairs + 12.0 * n
     out = {
@@ -158,7 +275,7 @@ def main():
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": ms_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic (reference exponential generator, seed 0)",
+        "dtype": "f64" if refine_used != "mfma" else "bf16 hi/lo (f32 acc) + f64 band recheck", "data": "synthetic (reference exponential generator, seed 0)",
         "config": {"workload": cfg["name"], "n": n, "dim": dim, "indexed_dims": k,
                    "eps": eps, "pairs": pairs, "total_candidates": total_cand,
                    "cells": iinfo["nNonEmptyCells"], "cell0_silenced": iinfo["n_cell0_queries"],
@@ -167,10 +284,11 @@ def main():
         "pair_hash": "%016x" % infos[0]["pair_hash"],
         "join_kernel_ms": kernel_ms,
         "estimate_ms": infos[-1]["estimate_ms"],
-        "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved_tf / PEAK_FP64_TFLOPS, "traffic": None,
-                     "note": "FP64 compute bound: 3*dim flops (sub + fma per dim) per "
-                             "reference candidate; FP64 vector and matrix peaks are equal"},
+        "refine": refine_used,
+        "band_rechecks": infos[-1]["band_rechecks"],
+        "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": peak_tf,
+                     "unit": "TFLOP/s", "frac": achieved_tf / peak_tf, "traffic": None,
+                     "note": roof_note},
         "roofline_hbm": {"bound": "hbm", "achieved": bytes_alg / (kernel_ms * 1e-3) / 1e9,
                          "peak": PEAK_HBM_GBS, "unit": "GB/s",
                          "frac": bytes_alg / (kernel_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,

@@ -73,16 +73,39 @@ class _ResultInfo(C.Structure):
                 ("n_batches", C.c_uint32), ("n_units", C.c_uint32), ("n_deferred", C.c_uint32),
                 ("estimate_ms", C.c_double), ("join_ms", C.c_double), ("workload_ms", C.c_double),
                 ("total_ms", C.c_double), ("join_kernel_avg_ms", C.c_double),
-                ("join_launches", C.c_uint32), ("reserved", C.c_uint32)]
+                ("join_launches", C.c_uint32), ("refine_used", C.c_uint32)]
+
+
+class _ServeOpts(C.Structure):
+    _fields_ = [("dim", C.c_uint32), ("indexed_dims", C.c_uint32), ("n_levels", C.c_uint32),
+                ("slots", C.c_uint32), ("cell_sides", C.c_double * 8),
+                ("result_bytes", C.c_uint64), ("max_queries", C.c_uint32), ("device", C.c_int32)]
+
+
+class _RequestInfo(C.Structure):
+    _fields_ = [("ticket", C.c_uint64), ("status", C.c_int32), ("level", C.c_uint32),
+                ("n_queries", C.c_uint32), ("reserved", C.c_uint32), ("pairs", C.c_uint64),
+                ("pair_hash", C.c_uint64), ("evals", C.c_uint64), ("result_bytes", C.c_uint64),
+                ("queue_ms", C.c_double), ("count_ms", C.c_double), ("admit_ms", C.c_double),
+                ("fill_ms", C.c_double), ("latency_ms", C.c_double)]
+
+
+class _ServeStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("pairs", C.c_uint64), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
+                ("max_ms", C.c_double), ("mean_ms", C.c_double), ("pool_bytes", C.c_uint64),
+                ("pool_in_use", C.c_uint64), ("pool_peak", C.c_uint64), ("wall_ms", C.c_double)]
 
 
 def _struct_dict(s):
     out = {}
     for name, _ in s._fields_:
         v = getattr(s, name)
-        if name in ("reserved",):
+        if name == "reserved":
             continue
-        if hasattr(v, "__len__"):
+        if name == "refine_used":
+            v = {1: "fp64", 2: "mfma"}.get(v, "none")
+        if isinstance(v, C.Array):
             v = list(v)
         out[name] = v
     return out
@@ -111,6 +134,13 @@ _lib.hegj_result_free.argtypes = [_vp]
 _lib.hegj_result_get_info.argtypes = [_vp, C.POINTER(_ResultInfo)]
 _lib.hegj_result_copy_counts.argtypes = [_vp, _vp]
 _lib.hegj_result_copy_csr.argtypes = [_vp, _vp, _vp, C.c_int]
+_lib.hegj_serve_open.argtypes = [_vp, C.c_uint32, C.POINTER(_ServeOpts), C.POINTER(_vp)]
+_lib.hegj_serve_submit.argtypes = [_vp, _vp, C.c_uint32, C.c_double, C.POINTER(C.c_uint64)]
+_lib.hegj_serve_wait.argtypes = [_vp, C.c_uint64, C.POINTER(_RequestInfo)]
+_lib.hegj_serve_copy.argtypes = [_vp, C.c_uint64, _vp, _vp]
+_lib.hegj_serve_release.argtypes = [_vp, C.c_uint64]
+_lib.hegj_serve_get_stats.argtypes = [_vp, C.POINTER(_ServeStats), C.c_int]
+_lib.hegj_serve_close.argtypes = [_vp]
 
 
 def _check(status):
@@ -302,6 +332,90 @@ class NeighborTable:
         _check(_lib.hegj_result_copy_csr(self._h, off.ctypes.data, nb.ctypes.data,
                                          1 if sort_rows else 0))
         return off, nb[:total]
+
+
+class Server:
+    """Device-resident index answering range-join requests (continuous
+    batching over `slots` streams, admission against a result-buffer pool of
+    `result_bytes`).  A request (queries (nq, dim), eps) returns, per query,
+    the ids of every point p with sqrt(fma-chain(p - q)) <= eps.
+
+    cell_sides: ascending grid levels; a request runs on the smallest cell
+    side >= eps (times 1 - 2^-30)."""
+
+    def __init__(self, database, cell_sides, indexed_dims=None, slots=4,
+                 result_bytes=1 << 30, max_queries=1 << 20, device=0):
+        arr = np.ascontiguousarray(database, dtype=np.float64)
+        if arr.ndim != 2:
+            raise ValueError("database must be (n, dim)")
+        n, dim = arr.shape
+        o = _ServeOpts()
+        o.dim = dim
+        o.indexed_dims = indexed_dims if indexed_dims is not None else min(dim, 6)
+        sides = list(cell_sides)
+        if not 1 <= len(sides) <= 8:
+            raise ValueError("1 to 8 cell sides")
+        o.n_levels = len(sides)
+        for i, c in enumerate(sides):
+            o.cell_sides[i] = float(c)
+        o.slots = slots
+        o.result_bytes = int(result_bytes)
+        o.max_queries = int(max_queries)
+        o.device = device
+        self._h = _vp()
+        self.dim = dim
+        _check(_lib.hegj_serve_open(arr.ctypes.data, n, C.byref(o), C.byref(self._h)))
+
+    def submit(self, queries, eps):
+        """Queue a request; returns its ticket (the queries are copied)."""
+        q = np.ascontiguousarray(queries, dtype=np.float64)
+        if q.ndim != 2 or q.shape[1] != self.dim:
+            raise ValueError("queries must be (nq, dim)")
+        t = C.c_uint64()
+        _check(_lib.hegj_serve_submit(self._h, q.ctypes.data, q.shape[0], float(eps), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket):
+        """Block until the request is done; returns its info dict (raises
+        HegjError when the request failed)."""
+        info = _RequestInfo()
+        _check(_lib.hegj_serve_wait(self._h, ticket, C.byref(info)))
+        return _struct_dict(info)
+
+    def result(self, ticket, release=True):
+        """(offsets[nq+1], neighbors[pairs]) of a finished request."""
+        info = self.wait(ticket)
+        off = np.zeros(info["n_queries"] + 1, np.uint64)
+        nb = np.zeros(max(info["pairs"], 1), np.uint32)
+        _check(_lib.hegj_serve_copy(self._h, ticket, off.ctypes.data, nb.ctypes.data))
+        if release:
+            self.release(ticket)
+        return off, nb[:info["pairs"]]
+
+    def release(self, ticket):
+        _check(_lib.hegj_serve_release(self._h, ticket))
+
+    def stats(self, reset=False):
+        st = _ServeStats()
+        _check(_lib.hegj_serve_get_stats(self._h, C.byref(st), 1 if reset else 0))
+        return _struct_dict(st)
+
+    def close(self):
+        if self._h:
+            _lib.hegj_serve_close(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def selfjoin(database, epsilon, indexed_dims=None, emulate_cell0=True, device=0, **kw):

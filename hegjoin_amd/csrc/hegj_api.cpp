@@ -27,7 +27,18 @@ hegj_status fail(hegj_status s, const std::string &msg) {
     return s;
 }
 
+static thread_local Arena *g_arena = nullptr;
+
+void set_thread_arena(Arena *a) { g_arena = a; }
+
 hipError_t Scratch::alloc(void **p, size_t bytes) {
+    if (g_arena) {
+        const size_t off = (g_arena->used + 255) & ~(size_t)255;
+        if (off + bytes > g_arena->cap) return hipErrorOutOfMemory;
+        *p = g_arena->base + off;
+        g_arena->used = off + bytes;
+        return hipSuccess;
+    }
     hipError_t e = hipMalloc(p, bytes);
     if (e == hipSuccess) ptrs.push_back(*p);
     return e;
@@ -39,7 +50,7 @@ Scratch::~Scratch() {
 
 // S7: largest s with RN(sqrt(s)) <= eps.  sqrt is monotone, so
 // sqrt(acc) <= eps  <=>  acc <= T, and the device needs no sqrt.
-static double sqrt_threshold(double eps) {
+double sqrt_threshold(double eps) {
     double s = eps * eps;
     const double inf = std::numeric_limits<double>::infinity();
     while (s > 0 && std::sqrt(s) > eps) s = std::nextafter(s, 0.0);
@@ -89,6 +100,11 @@ struct hegj_index {
     uint64_t total_candidates = 0;
     double workload_ms = 0;
     bool have_workload = false;
+    // split-BF16 operands of the MFMA refine (lazily computed)
+    uint16_t *gb_A = nullptr, *gb_B = nullptr;
+    float *gb_w = nullptr;
+    uint32_t gb_kp = 0;
+    double gb_ms = 0;
 };
 
 struct hegj_result {
@@ -193,7 +209,66 @@ hegj_status ensure_morton(hegj_index *h) {
     return morton_order(h->g, h->ix, h->d_morton, &h->morton_ok, h->stream);
 }
 
+hegj_status ensure_gb(hegj_index *h) {
+    if (h->gb_A) return HEGJ_OK;
+    const uint32_t kb = gb_kblocks(h->g.dim);
+    if (kb == 0) return fail(HEGJ_ERR_UNSUPPORTED, "no MFMA refine for this dimensionality");
+    EventTimer t(h->stream);
+    HEGJ_TRY(gb_prepare(h->g, h->ix, kb * 32, &h->gb_A, &h->gb_B, &h->gb_w, h->stream));
+    h->gb_kp = kb * 32;
+    h->gb_ms = t.stop();
+    return HEGJ_OK;
+}
+
+// Join through the MFMA guard-band refine: one pass, the pair list sized on
+// the device, the band rechecked in FP64.  Same pair set as the FP64 kernel.
+hegj_status selfjoin_mfma(hegj_index *h, const uint32_t *order, hegj_result *r) {
+    hipStream_t s = h->stream;
+    const uint32_t n = h->g.n;
+    HEGJ_TRY(ensure_gb(h));
+    Scratch sc;
+    uint2 *units = nullptr;
+    uint32_t U = 0;
+    HEGJ_TRY(build_units(h->g, h->ix, gb_unit_queries(h->gb_kp / 32), order, &units, &U, s));
+    sc.ptrs.push_back(units);
+    r->info.n_units = U;
+    HIP_TRY(hipMalloc(&r->d_row_off, sizeof(uint64_t) * n));
+    HIP_TRY(hipMalloc(&r->d_row_cnt, sizeof(uint32_t) * n));
+    HIP_TRY(hipMemsetAsync(r->d_row_off, 0, sizeof(uint64_t) * n, s));
+    HIP_TRY(hipMemsetAsync(r->d_row_cnt, 0, sizeof(uint32_t) * n, s));
+    unsigned long long *d_hash;
+    HIP_TRY(sc.get(&d_hash, 1));
+    HIP_TRY(hipMemsetAsync(d_hash, 0, sizeof(*d_hash), s));
+    GbOutput o;
+    o.row_off = r->d_row_off;
+    o.row_cnt = r->d_row_cnt;
+    o.hash = d_hash;
+    HEGJ_TRY(gb_join(h->g, h->ix, units, U, h->gb_A, h->gb_B, h->gb_w, h->gb_kp, h->device, s, &o));
+    r->batches.push_back(o.buf);
+    r->bases.push_back(0);
+    unsigned long long hash = 0;
+    HIP_TRY(hipMemcpyAsync(&hash, d_hash, sizeof(hash), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    r->info.total_neighbors = o.pairs;
+    r->info.pair_hash = hash;
+    r->info.evals_executed = o.evals;
+    r->info.band_rechecks = o.band;
+    r->info.n_batches = 1;
+    r->info.estimate_ms = h->gb_ms;
+    r->info.join_ms = o.kernel_ms;
+    r->info.join_launches = 1;
+    r->info.join_kernel_avg_ms = o.kernel_ms;
+    r->info.refine_used = HEGJ_REFINE_MFMA;
+    return HEGJ_OK;
+}
+
 }  // namespace
+
+namespace hegj {
+const GridGeom &index_geom(const hegj_index *h) { return h->g; }
+const DevIndex &index_dev(const hegj_index *h) { return h->ix; }
+hegj_status select_device(int device) { return set_device(device); }
+}  // namespace hegj
 
 extern "C" {
 
@@ -271,6 +346,9 @@ hegj_status hegj_index_free(hegj_index *h) {
     if (h->d_work) (void)hipFree(h->d_work);
     if (h->d_order) (void)hipFree(h->d_order);
     if (h->d_morton) (void)hipFree(h->d_morton);
+    if (h->gb_A) (void)hipFree(h->gb_A);
+    if (h->gb_B) (void)hipFree(h->gb_B);
+    if (h->gb_w) (void)hipFree(h->gb_w);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return HEGJ_OK;
@@ -375,8 +453,13 @@ hegj_status hegj_selfjoin(hegj_index *h, const hegj_join_opts *opts_in, hegj_res
         opts = *opts_in;
     else
         hegj_join_opts_default(&opts);
+    if (opts.refine > HEGJ_REFINE_MFMA) return fail(HEGJ_ERR_INVALID_ARG, "unknown refine");
+    const bool mfma = opts.refine == HEGJ_REFINE_MFMA ||
+                      (opts.refine == HEGJ_REFINE_AUTO && h->g.dim >= 16 && gb_kblocks(h->g.dim) != 0);
+    if (mfma && gb_kblocks(h->g.dim) == 0)
+        return fail(HEGJ_ERR_UNSUPPORTED, "no MFMA refine for this dimensionality");
     const uint32_t qb = opts.queries_per_unit ? opts.queries_per_unit : default_qb(h->g.dim);
-    if (qb != 4 && qb != 8 && qb != 16)
+    if (!mfma && qb != 4 && qb != 8 && qb != 16)
         return fail(HEGJ_ERR_INVALID_ARG, "queries_per_unit must be 4, 8 or 16");
     hipStream_t s = h->stream;
     const uint32_t n = h->g.n;
@@ -400,6 +483,14 @@ hegj_status hegj_selfjoin(hegj_index *h, const hegj_join_opts *opts_in, hegj_res
     } else if (opts.unit_order != HEGJ_ORDER_SPATIAL) {
         return fail(HEGJ_ERR_INVALID_ARG, "unknown unit_order");
     }
+    if (mfma) {
+        HEGJ_TRY(selfjoin_mfma(h, order, r.get()));
+        r->info.total_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+        *out = r.release();
+        return HEGJ_OK;
+    }
+    r->info.refine_used = HEGJ_REFINE_FP64;
     HEGJ_TRY(build_units(h->g, h->ix, qb, order, &units, &U, s));
     sc.ptrs.push_back(units);
     r->info.n_units = U;
@@ -467,7 +558,8 @@ hegj_status hegj_selfjoin(hegj_index *h, const hegj_join_opts *opts_in, hegj_res
     uint64_t capacity = opts.result_capacity;
     if (capacity == 0 && U > 0) {
         EventTimer t(s);
-        uint32_t stride = std::max<uint32_t>(1, U / 65536);
+        // ~8192 sampled units: a few per cent of the join's work at most
+        uint32_t stride = std::max<uint32_t>(1, U / 8192);
         uint32_t ns = (U + stride - 1) / stride;
         uint2 *sample;
         HIP_TRY(sc.get(&sample, ns));
@@ -510,11 +602,9 @@ hegj_status hegj_selfjoin(hegj_index *h, const hegj_join_opts *opts_in, hegj_res
         r->bases.push_back(base);
         // reset per-batch counters (hash / evals / pairs accumulate)
         HIP_TRY(hipMemsetAsync(d_ctr, 0, offsetof(Counters, hash), s));
-        a.units_per_grab = grab_for(parts > 1 && cur_n == U ? cur_n / parts : cur_n);
-        if (opts.units_per_grab) a.units_per_grab = opts.units_per_grab;
         a.units = cur;
         a.n_units = cur_n;
-        a.units_per_grab = grab_for(cur_n);
+        a.units_per_grab = opts.units_per_grab ? opts.units_per_grab : grab_for(cur_n);
         if (cur == units && cur_n == U) {
             set_parts(a, cur_n, parts, part_end);
         } else {

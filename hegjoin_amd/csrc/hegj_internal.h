@@ -70,6 +70,15 @@ struct DevIndex {
     uint32_t C = 0, R = 0;
 };
 
+// Device bump allocator owned by one host thread.  While installed for that
+// thread, Scratch carves temporaries from it instead of calling hipMalloc /
+// hipFree (which synchronise the whole device); the owner resets `used`.
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+};
+void set_thread_arena(Arena *a);  // nullptr: back to hipMalloc
+
 // Scratch allocator for temporaries of one call (freed by the destructor).
 struct Scratch {
     std::vector<void *> ptrs;
@@ -80,6 +89,9 @@ struct Scratch {
     }
     ~Scratch();
 };
+
+// Largest s with RN(sqrt(s)) <= eps (the distance predicate without sqrt).
+double sqrt_threshold(double eps);
 
 // ---- primitives (hegj_prims.hip) -----------------------------------------
 // Exclusive scan; writes the total to *d_total when non-null.
@@ -153,6 +165,46 @@ hegj_status gb_prepare(const GridGeom &g, const DevIndex &ix, uint32_t kp, uint1
 hegj_status gb_join(const GridGeom &g, const DevIndex &ix, const uint2 *units, uint32_t n_units,
                     const uint16_t *Av, const uint16_t *Bv, const float *w, uint32_t kp,
                     int device, hipStream_t s, GbOutput *out);
+
+// ---- query join against a resident index (hegj_query.hip) ----------------
+struct QueryGeom {
+    GridGeom g;          // the index level (g.eps = cell side)
+    uint64_t dead_key;   // key of queries with no cell in reach (sorts last)
+    int key_bits;
+};
+struct QueryPlan {
+    uint64_t *keys = nullptr;    // [nq] sorted widened cell keys
+    uint32_t *ids = nullptr;     // [nq] query ids in key order
+    uint32_t *starts = nullptr;  // [n_units + 1] first sorted slot of each unit
+    uint32_t nq = 0, n_units = 0, n_live = 0;
+};
+struct QJoinArgs {
+    QueryGeom qg;
+    DevIndex ix;
+    const double *queries;       // [nq * dim] row-major
+    const uint64_t *keys;
+    const uint32_t *ids;
+    const uint32_t *starts;
+    uint32_t n_units;
+    uint32_t *queue_head;        // zeroed before each launch
+    double thresh;               // largest s with sqrt(s) <= eps of the request
+    uint32_t *counts;            // [nq] (count pass)
+    const uint64_t *offsets;     // [nq + 1] (fill pass)
+    uint32_t *out;               // neighbour ids (fill pass)
+    unsigned long long *hash;
+    unsigned long long *evals;
+};
+hegj_status query_geom(const GridGeom &g, QueryGeom *qg);
+// Accessors of an index handle (hegj_api.cpp).
+const GridGeom &index_geom(const hegj_index *h);
+const DevIndex &index_dev(const hegj_index *h);
+hegj_status select_device(int device);
+// Sort the queries by cell and build the units; the plan's arrays come from `keep`.
+hegj_status query_plan(QueryPlan *p, Scratch &keep, const QueryGeom &qg, const double *queries,
+                       uint32_t nq, hipStream_t s);
+hegj_status query_run(const QJoinArgs &a, bool fill, int device, hipStream_t s);
+// counts[nq] -> offsets[nq + 1] (offsets[nq] = total)
+hegj_status query_offsets(const uint32_t *counts, uint32_t nq, uint64_t *offsets, hipStream_t s);
 
 // ---- utilities (hegj_util.hip) -------------------------------------------
 hegj_status workload_order(const DevIndex &ix, const uint32_t *work, uint32_t *order,

@@ -36,6 +36,10 @@ namespace hegj {
 namespace {
 
 constexpr int kWavesPerBlock = 4;
+constexpr int kWavesPerBlockQ = kWavesPerBlock;
+// dynamic LDS for staged query coordinates (with the static row lists and
+// hit masks the workgroup stays within 64 KiB)
+constexpr uint32_t kMaxQueryLds = 36u << 10;
 
 // Compacted stencil-row list of one unit in LDS.
 struct RowList {
@@ -116,38 +120,55 @@ __device__ __forceinline__ uint32_t cand_pos_adv(const RowList &rl, uint32_t nsl
 // Candidate coordinates: D doubles from the cell-ordered SoA copy.
 template <int D>
 __device__ __forceinline__ void load_cand(const JoinArgs &a, uint32_t pos, bool valid, double *x) {
+    // unconditional loads (invalid lanes read position 0, always in range, and
+    // are masked by the caller): no branch, so the compiler can wait for
+    // exactly the chunk it needs instead of every load in flight
     const uint64_t n = a.g.n;
+    const uint32_t p = valid ? pos : 0u;
 #pragma unroll
-    for (int l = 0; l < D; l++) x[l] = valid ? a.ix.soa[(uint64_t)l * n + pos] : 0.0;
+    for (int l = 0; l < D; l++) x[l] = a.ix.soa[(uint64_t)l * n + p];
 }
 
 // acc[qi] for one candidate (coordinates in x) against the unit's queries:
 // acc = fma(c - q, c - q, acc) in dimension order (kernel.cu:350-353 with
-// FMA contraction).  Query coordinates are wave-uniform loads.
+// FMA contraction).  q holds the unit's query coordinates, loaded once per
+// unit (wave-uniform: scalar registers or LDS), so the candidate loads that
+// are in flight are never waited on here.
+// All QB queries are evaluated (padding queries hold zeros and are masked by
+// the caller): no branch between the candidate loads and their use.
 template <int D, int QB>
-__device__ __forceinline__ void dist_regs(const JoinArgs &a, const double *x, uint32_t qpos0,
-                                          uint32_t nq, double *acc) {
-    const uint64_t n = a.g.n;
-    const double *__restrict__ soa = a.ix.soa;
+__device__ __forceinline__ void dist_regs(const double *x, const double (*q)[D], double *acc) {
 #pragma unroll
     for (int qi = 0; qi < QB; qi++) {
         acc[qi] = 0.0;
-        if ((uint32_t)qi < nq) {
-            const uint64_t qp = qpos0 + qi;
 #pragma unroll
-            for (int l = 0; l < D; l++) {
-                double dd = x[l] - soa[(uint64_t)l * n + qp];
-                acc[qi] = __builtin_fma(dd, dd, acc[qi]);
-            }
+        for (int l = 0; l < D; l++) {
+            double dd = x[l] - q[qi][l];
+            acc[qi] = __builtin_fma(dd, dd, acc[qi]);
+        }
+    }
+}
+
+// Same with the query coordinates in LDS (qs[qi * dim + l]).
+template <int D, int QB>
+__device__ __forceinline__ void dist_regs_lds(const double *x, const double *qs, double *acc) {
+#pragma unroll
+    for (int qi = 0; qi < QB; qi++) {
+        acc[qi] = 0.0;
+#pragma unroll
+        for (int l = 0; l < D; l++) {
+            double dd = x[l] - qs[qi * D + l];
+            acc[qi] = __builtin_fma(dd, dd, acc[qi]);
         }
     }
 }
 
 // Generic dimension count (runtime or large D): candidate coordinates are
-// streamed in tiles of 16 dims, acc keeps the dimension order.
+// streamed in tiles of 16 dims, acc keeps the dimension order.  The unit's
+// query coordinates come from LDS (qs[qi * dim + l]).
 template <int D, int QB>
 __device__ __forceinline__ void dist_stream(const JoinArgs &a, uint32_t pos, bool valid,
-                                            uint32_t qpos0, uint32_t nq, double *acc) {
+                                            const double *qs, uint32_t nq, double *acc) {
     const uint32_t dim = D > 0 ? (uint32_t)D : a.g.dim;
     const uint64_t n = a.g.n;
     const double *__restrict__ soa = a.ix.soa;
@@ -164,18 +185,29 @@ __device__ __forceinline__ void dist_stream(const JoinArgs &a, uint32_t pos, boo
 #pragma unroll
         for (int qi = 0; qi < QB; qi++) {
             if ((uint32_t)qi < nq) {
-                const uint64_t qp = qpos0 + qi;
 #pragma unroll
                 for (int i = 0; i < DT; i++) {
                     uint32_t l = d0 + i;
                     if (l < dim) {
-                        double dd = x[i] - soa[(uint64_t)l * n + qp];
+                        double dd = x[i] - qs[qi * dim + l];
                         acc[qi] = __builtin_fma(dd, dd, acc[qi]);
                     }
                 }
             }
         }
     }
+}
+
+// Query coordinates of a unit kept in scalar registers (small QB * D);
+// otherwise they are staged in LDS, QB * dim doubles per wave.
+template <int D, int QB>
+constexpr bool q_in_sgprs() {
+    return D > 0 && D <= 32 && QB * D <= 48;
+}
+
+template <int D, int QB>
+constexpr uint32_t q_lds_bytes(uint32_t dim) {
+    return q_in_sgprs<D, QB>() ? 0u : (uint32_t)(kWavesPerBlockQ * QB * (D > 0 ? D : dim) * sizeof(double));
 }
 
 // Hit masks of one unit: mask[qi * nchunks + ch], one 64-bit ballot per
@@ -213,6 +245,7 @@ template <int D, int QB>
 __global__ __launch_bounds__(64 * kWavesPerBlock) void k_join(JoinArgs a) {
     __shared__ RowList rls[kWavesPerBlock];
     __shared__ HitStore hs_all[kWavesPerBlock];
+    extern __shared__ double q_dyn[];  // QB * dim query coordinates per wave
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     RowList &rl = rls[wid];
     uint64_t *const hs_lds = hs_all[wid].w;
@@ -221,6 +254,10 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_join(JoinArgs a) {
     constexpr bool kRegs = (D > 0 && D <= kRegDims);
     constexpr int DR = kRegs ? D : 1;
     uint64_t my_hash = 0, my_evals = 0, my_pairs = 0;
+    constexpr bool kQS = q_in_sgprs<D, QB>();
+    const uint32_t dimq = D > 0 ? (uint32_t)D : a.g.dim;
+    double *const qs = q_dyn + (size_t)wid * QB * dimq;
+    double qreg[kQS ? QB : 1][kQS ? DR : 1];
     uint32_t part = (a.part_by_xcc ? xcc_id() : blockIdx.x) % a.n_parts;
     uint32_t tries = 0, grab = 0, gend = 0;
 
@@ -235,6 +272,31 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_join(JoinArgs a) {
             uint32_t nslots;
             bool live;
             const uint32_t M = build_rows(a, cell, rl, &nslots, &live);
+            // the unit's query coordinates, once: lane i loads element i
+            if (kQS) {
+                constexpr int NQD = kQS ? QB * DR : 1;
+#pragma unroll
+                for (int i0 = 0; i0 < NQD; i0 += 64) {
+                    const int i = i0 + lane;
+                    const int qi = i / DR, l = i % DR;
+                    double v = 0.0;
+                    if (i < NQD && (uint32_t)qi < nq)
+                        v = a.ix.soa[(uint64_t)l * a.g.n + qpos0 + qi];
+#pragma unroll
+                    for (int j = i0; j < NQD && j < i0 + 64; j++) {
+                        const uint64_t u = (uint64_t)__double_as_longlong(v);
+                        const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, j - i0);
+                        const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), j - i0);
+                        qreg[kQS ? j / DR : 0][kQS ? j % DR : 0] =
+                            __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+                    }
+                }
+            } else {
+                for (uint32_t i = lane; i < QB * dimq; i += 64) {
+                    const uint32_t qi = i / dimq, l = i - qi * dimq;
+                    qs[i] = qi < nq ? a.ix.soa[(uint64_t)l * a.g.n + qpos0 + qi] : 0.0;
+                }
+            }
             wave_lds_sync();
             const uint32_t nchunks = (M + 63) / 64;
             const uint32_t nmask = nq * nchunks;
@@ -258,39 +320,44 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_join(JoinArgs a) {
                     load_cand<DR>(a, v ? cand_pos_adv(rl, nslots, t, slot) : 0u, v, ring[p]);
                 }
             }
+            // Whole groups of kDepth chunks: the chunks past the end of the
+            // last group have no valid lane (their ballots are empty) and
+            // their prefetches read position 0, so the loop body has no
+            // branch around a load and the compiler waits for the oldest
+            // chunk only.
             for (uint32_t ch0 = 0; ch0 < nchunks; ch0 += kDepth) {
 #pragma unroll
                 for (int p = 0; p < kDepth; p++) {
                     const uint32_t ch = ch0 + p;
-                    if (ch >= nchunks) break;
+                    if (!kRegs && ch >= nchunks) break;
                     double acc[QB];
                     const bool valid = ch * 64 + lane < M;
                     if (kRegs) {
-                        double x[DR];
-#pragma unroll
-                        for (int l = 0; l < DR; l++) x[l] = ring[p][l];
-                        if (ch + kDepth < nchunks) {
+                        // use ring[p], then refill it with chunk ch + kDepth
+                        // (no copy of registers that still have loads in flight)
+                        if constexpr (kQS)
+                            dist_regs<DR, QB>(ring[p], qreg, acc);
+                        else
+                            dist_regs_lds<DR, QB>(ring[p], qs, acc);
+                        {
                             const uint32_t tn = (ch + kDepth) * 64 + lane;
                             const bool vn = tn < M;
                             load_cand<DR>(a, vn ? cand_pos_adv(rl, nslots, tn, slot) : 0u, vn,
                                           ring[p]);
                         }
-                        dist_regs<DR, QB>(a, x, qpos0, nq, acc);
                     } else {
                         const uint32_t t = ch * 64 + lane;
                         const uint32_t pos = valid ? cand_pos_adv(rl, nslots, t, slot) : 0u;
-                        dist_stream<D, QB>(a, pos, valid, qpos0, nq, acc);
+                        dist_stream<D, QB>(a, pos, valid, qs, nq, acc);
                     }
                     uint64_t mine = 0;  // lane qi keeps query qi's mask
 #pragma unroll
                     for (int qi = 0; qi < QB; qi++) {
-                        if ((uint32_t)qi < nq) {
-                            const uint64_t m = __ballot(valid && acc[qi] <= T);
-                            cnt[qi] += (uint32_t)__popcll(m);
-                            if (lane == qi) mine = m;
-                        }
+                        const uint64_t m = __ballot(valid && (uint32_t)qi < nq && acc[qi] <= T);
+                        cnt[qi] += (uint32_t)__popcll(m);
+                        mine = lane == qi ? m : mine;
                     }
-                    if (keep && (uint32_t)lane < nq) hs[lane * nchunks + ch] = mine;
+                    if (keep && (uint32_t)lane < nq && ch < nchunks) hs[lane * nchunks + ch] = mine;
                 }
             }
             my_evals += (uint64_t)M * nq;
@@ -344,9 +411,19 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_join(JoinArgs a) {
                     } else {
                         const bool valid = t < M;
                         const uint32_t pos = valid ? cand_pos(rl, nslots, t) : 0u;
-                        double acc[1];
-                        dist_stream<D, 1>(a, pos, valid, qpos0 + qi, 1, acc);
-                        m = __ballot(valid && acc[0] <= T);
+                        double acc1 = 0.0;
+                        if (kQS) {
+                            double x[DR];
+                            load_cand<DR>(a, pos, valid, x);
+#pragma unroll
+                            for (int l = 0; l < DR; l++) {
+                                const double dd = x[l] - qreg[kQS ? qi : 0][l];
+                                acc1 = __builtin_fma(dd, dd, acc1);
+                            }
+                        } else {
+                            dist_stream<D, 1>(a, pos, valid, qs + qi * dimq, 1, &acc1);
+                        }
+                        m = __ballot(valid && acc1 <= T);
                         my_evals += (uint64_t)min(64u, M - ch * 64);
                     }
                     if (m == 0) continue;
@@ -453,7 +530,10 @@ hegj_status build_units(const GridGeom &g, const DevIndex &ix, uint32_t qb, cons
 
 template <int D, int QB>
 static hegj_status launch_join_t(const JoinArgs &a, uint32_t blocks, hipStream_t s) {
-    k_join<D, QB><<<blocks, 64 * kWavesPerBlock, 0, s>>>(a);
+    const uint32_t lds = q_lds_bytes<D, QB>(a.g.dim);
+    if (lds > kMaxQueryLds)
+        return fail(HEGJ_ERR_UNSUPPORTED, "queries_per_unit * dim too large for LDS; lower queries_per_unit");
+    k_join<D, QB><<<blocks, 64 * kWavesPerBlock, lds, s>>>(a);
     HIP_TRY(hipGetLastError());
     return HEGJ_OK;
 }
@@ -485,7 +565,8 @@ namespace hegj {
 hegj_status join_occupancy(uint32_t dim, uint32_t qb, int device, uint32_t *blocks) {
     int per_cu = 0;
     const void *fn = nullptr;
-#define HEGJ_OCC(D, Q) fn = reinterpret_cast<const void *>(&k_join<D, Q>)
+    uint32_t lds = 0;
+#define HEGJ_OCC(D, Q) fn = reinterpret_cast<const void *>(&k_join<D, Q>), lds = q_lds_bytes<D, Q>(dim)
     uint32_t d = (dim == 2 || dim == 6 || dim == 18 || dim == 90) ? dim : 0;
     switch (qb * 1000 + d) {
         case 4002: HEGJ_OCC(2, 4); break;
@@ -506,7 +587,9 @@ hegj_status join_occupancy(uint32_t dim, uint32_t qb, int device, uint32_t *bloc
         default: return fail(HEGJ_ERR_INVALID_ARG, "queries_per_unit must be 4, 8 or 16");
     }
 #undef HEGJ_OCC
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * kWavesPerBlock, 0));
+    if (lds > kMaxQueryLds)
+        return fail(HEGJ_ERR_UNSUPPORTED, "queries_per_unit * dim too large for LDS; lower queries_per_unit");
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * kWavesPerBlock, lds));
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, device));
     if (per_cu < 1) per_cu = 1;

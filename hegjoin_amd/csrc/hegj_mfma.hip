@@ -354,17 +354,15 @@ __global__ __launch_bounds__(64 * kGbWaves) void k_gb_join(GbArgs a) {
                     if (v) {
                         stage[i] = src[u4];
                     } else {
-                        // an absent candidate: s' = 2^100 against every query
-                        uint32_t e = a.g.dim + 1 - u4 * 8;  // element of ndn in this unit
-                        uint4 z = make_uint4(0, 0, 0, 0);
-                        if (e < 8) {
-                            uint32_t word = 0x7180u << ((e & 1) * 16);
-                            if (e / 2 == 0) z.x = word;
-                            else if (e / 2 == 1) z.y = word;
-                            else if (e / 2 == 2) z.z = word;
-                            else z.w = word;
+                        // an absent candidate: 2^100 at elements dim and dim+1,
+                        // so s' >= 2^100 (or +inf) against any query row
+                        uint32_t wd[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+                        for (uint32_t x = 0; x < 2; x++) {
+                            const uint32_t e = a.g.dim + x - u4 * 8;
+                            if (e < 8) wd[e >> 1] |= 0x7180u << ((e & 1) * 16);
                         }
-                        stage[i] = z;
+                        stage[i] = make_uint4(wd[0], wd[1], wd[2], wd[3]);
                     }
                 }
             }
@@ -434,11 +432,13 @@ __global__ __launch_bounds__(64 * kGbWaves) void k_gb_join(GbArgs a) {
                             const uint32_t i = i0 + lane;
                             uint64_t key = 0;
                             bool acc_ok = false, band_ok = false;
-                            if (i < ns) {
-                                const uint2 e = surv[wid][i];
-                                const uint32_t j = e.x >> 6, src = e.x & 63;
-                                const uint32_t q = wq0 + rt * 16 + (src >> 4) * 4 + j;
-                                const uint32_t ci = ct * 16 + (src & 15);
+                            const uint2 e = i < ns ? surv[wid][i] : make_uint2(0u, 0u);
+                            const uint32_t j = e.x >> 6, src = e.x & 63;
+                            const uint32_t q = wq0 + rt * 16 + (src >> 4) * 4 + j;
+                            const uint32_t ci = ct * 16 + (src & 15);
+                            // padding rows/columns never pass (s' >= 2^100); the
+                            // bounds are checked here all the same
+                            if (i < ns && q < nq && ch * kGbChunk + ci < M) {
                                 const uint32_t qp = qpos0 + q, cp = cpos[buf][ci];
                                 key = ((uint64_t)qp << 32) | cp;
                                 const double hi = (double)__uint_as_float(e.y) + (double)a.w[qp] +

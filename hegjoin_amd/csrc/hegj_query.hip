@@ -1,0 +1,360 @@
+// hegj_query.hip -- range join of external query points against a resident
+// grid index: the device side of the serving loop.
+//
+// The reference has no query path (its only join is the self-join of
+// kernel.cu:727-802); this follows the same grid semantics with the intended
+// stencil (no cell-0 wrap) and the same distance chain and predicate:
+//   a query q and an index point p pair when sqrt(acc) <= eps, with
+//   acc = fma(p_l - q_l, p_l - q_l, acc) over l = 0..dim-1 (kernel.cu:350-355),
+//   evaluated as acc <= T, T the largest double with sqrt(T) <= eps.
+// The index level's cell side c satisfies eps <= c (1 - 2^-30), so every
+// point within eps lies in the 3^k cells around the query's cell even after
+// the rounding of (x - min) / c.
+//
+// Pipeline for one request (one stream):
+//   k_qcell_keys    query -> key of its cell in the grid widened by one cell
+//                   on each side (coordinates -1 .. nCells), or `dead` when
+//                   the query lies further out (no point can be in range);
+//   radix sort      (key, query id), dead keys last;
+//   k_qruns         one unit per run of equal keys (queries sharing a cell
+//                   share the candidate rows);
+//   k_qjoin<count>  per-query neighbour counts;
+//   scan            CSR offsets, exact result size (admission happens here);
+//   k_qjoin<fill>   each query's neighbours written at its own offset in
+//                   candidate (cell) order -- no atomics, deterministic.
+#include "hegj_internal.h"
+
+namespace hegj {
+
+namespace {
+
+constexpr int kQWaves = 4;
+constexpr int kQB = 4;  // queries evaluated together against one candidate chunk
+
+struct QRowList {
+    uint32_t start[kMaxRows];
+    uint32_t off[kMaxRows + 1];
+};
+
+__global__ __launch_bounds__(256) void k_qcell_keys(const double *__restrict__ q, uint32_t nq,
+                                                    QueryGeom qg, uint64_t *__restrict__ keys,
+                                                    uint32_t *__restrict__ ids) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nq) return;
+    uint64_t key = 0, mult = 1;
+    bool dead = false;
+    for (uint32_t j = 0; j < qg.g.k; j++) {
+        const double x = q[(uint64_t)i * qg.g.dim + j];
+        const double f = floor((x - qg.g.minArr[j]) / qg.g.eps);
+        // widened coordinate: -1 .. nCells map to 0 .. nCells + 1
+        if (!(f >= -1.0) || f > (double)qg.g.nCells[j]) {
+            dead = true;
+        } else {
+            key += (uint64_t)(int64_t)(f + 1.0) * mult;
+        }
+        mult *= (uint64_t)qg.g.nCells[j] + 2;
+    }
+    keys[i] = dead ? qg.dead_key : key;
+    ids[i] = i;
+}
+
+// Unit r starts at the r-th run of equal live keys; n_live = first dead slot.
+__global__ __launch_bounds__(256) void k_qrun_flags(const uint64_t *__restrict__ keys, uint32_t nq,
+                                                    uint64_t dead, uint32_t *__restrict__ flag,
+                                                    uint32_t *__restrict__ n_live) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nq) return;
+    const uint64_t k = keys[i];
+    flag[i] = (k != dead && (i == 0 || keys[i - 1] != k)) ? 1u : 0u;
+    const bool last_live = k != dead && (i + 1 == nq || keys[i + 1] == dead);
+    if (last_live) *n_live = i + 1;
+}
+
+__global__ __launch_bounds__(256) void k_qrun_fill(const uint32_t *__restrict__ flag,
+                                                   const uint32_t *__restrict__ rank, uint32_t nq,
+                                                   uint32_t *__restrict__ starts) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < nq && flag[i]) starts[rank[i]] = i;
+}
+
+// Stencil rows of the unit whose widened key is `key` into the wave's LDS
+// list; returns the candidate count.
+__device__ uint32_t qbuild_rows(const QJoinArgs &a, uint64_t key, QRowList &rl, uint32_t *nslots) {
+    const GridGeom &g = a.qg.g;
+    const int lane = threadIdx.x & 63;
+    uint32_t c[kMaxK], lo[kMaxK], hi[kMaxK];
+    bool ok = true;
+    for (uint32_t j = 0; j < g.k; j++) {
+        const uint64_t w = (uint64_t)g.nCells[j] + 2;
+        const uint64_t qq = key / w;
+        const int64_t cj = (int64_t)(key - qq * w) - 1;  // -1 .. nCells
+        key = qq;
+        const int64_t l = cj - 1 < 0 ? 0 : cj - 1;
+        const int64_t h = cj + 1 > (int64_t)g.nCells[j] - 1 ? (int64_t)g.nCells[j] - 1 : cj + 1;
+        ok &= l <= h;
+        c[j] = (uint32_t)cj;  // -1 wraps; stencil_row adds the offset modulo 2^32
+        lo[j] = (uint32_t)l;
+        hi[j] = (uint32_t)(h < 0 ? 0 : h);
+    }
+    if (!ok) {
+        *nslots = 0;
+        return 0;
+    }
+    const uint32_t nrows = pow3(g.k - 1);
+    uint32_t slots = 0, total = 0;
+    for (uint32_t r0 = 0; r0 < nrows; r0 += 64) {
+        const uint32_t r = r0 + lane;
+        uint32_t pstart = 0, plen = 0;
+        if (r < nrows) {
+            uint32_t cf, cl;
+            stencil_row(c, lo, hi, r, g, a.ix, &cf, &cl);
+            if (cl > cf) {
+                pstart = a.ix.cell_start[cf];
+                plen = a.ix.cell_start[cl] - pstart;
+            }
+        }
+        const uint64_t nz = __ballot(plen > 0);
+        uint32_t x = plen;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (plen > 0) {
+            const uint32_t s = slots + lanes_below(nz);
+            rl.start[s] = pstart;
+            rl.off[s] = total + x - plen;
+        }
+        slots += (uint32_t)__popcll(nz);
+        total += __shfl(x, 63, 64);
+    }
+    if (lane == 0) rl.off[slots] = total;
+    *nslots = slots;
+    return total;
+}
+
+// D > 0: coordinates in registers; D == 0: any dimensionality, one
+// dimension at a time.
+template <int D, bool FILL>
+__global__ __launch_bounds__(64 * kQWaves) void k_qjoin(QJoinArgs a) {
+    __shared__ QRowList rls[kQWaves];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    QRowList &rl = rls[wid];
+    const uint32_t dim = D > 0 ? (uint32_t)D : a.qg.g.dim;
+    const uint64_t n = a.qg.g.n;
+    const double T = a.thresh;
+    const double *__restrict__ soa = a.ix.soa;
+    uint64_t my_hash = 0, my_evals = 0;
+    while (true) {
+        uint32_t u = 0;
+        if (lane == 0) u = atomicAdd(a.queue_head, 1u);
+        u = __shfl(u, 0, 64);
+        if (u >= a.n_units) break;
+        const uint32_t s0 = a.starts[u], s1 = a.starts[u + 1];
+        uint32_t nslots;
+        const uint32_t M = qbuild_rows(a, a.keys[s0], rl, &nslots);
+        wave_lds_sync();
+        for (uint32_t g0 = s0; g0 < s1; g0 += kQB) {
+            const uint32_t nqg = min((uint32_t)kQB, s1 - g0);
+            uint32_t qid[kQB];
+            uint64_t base[kQB];
+            uint32_t cnt[kQB];
+#pragma unroll
+            for (int qi = 0; qi < kQB; qi++) {
+                qid[qi] = a.ids[g0 + min((uint32_t)qi, nqg - 1)];
+                base[qi] = FILL ? a.offsets[qid[qi]] : 0;
+                cnt[qi] = 0;
+            }
+            // the group's query coordinates, once (lane i loads element i;
+            // scalar registers), so no load in the candidate loop waits on
+            // anything but the candidates
+            constexpr int DQ = D > 0 ? D : 1;
+            double qreg[kQB][DQ];
+            if constexpr (D > 0) {
+                static_assert(kQB * D <= 64, "one load per lane");
+                const int qi = lane / DQ, l = lane % DQ;
+                double v = 0.0;
+                if (lane < kQB * DQ) {
+                    uint32_t id = 0;
+#pragma unroll
+                    for (int k = 0; k < kQB; k++) id = k == qi ? qid[k] : id;
+                    v = a.queries[(uint64_t)id * dim + l];
+                }
+#pragma unroll
+                for (int j = 0; j < kQB * DQ; j++) {
+                    const uint64_t u = (uint64_t)__double_as_longlong(v);
+                    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, j);
+                    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), j);
+                    qreg[j / DQ][j % DQ] = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+                }
+            }
+            uint32_t slot = 0;
+            for (uint32_t t0 = 0; t0 < M; t0 += 64) {
+                const uint32_t t = t0 + lane;
+                const bool valid = t < M;
+                uint32_t pos = 0;
+                if (valid) {
+                    while (slot + 1 < nslots && rl.off[slot + 1] <= t) slot++;
+                    pos = rl.start[slot] + (t - rl.off[slot]);
+                }
+                double acc[kQB];
+#pragma unroll
+                for (int qi = 0; qi < kQB; qi++) acc[qi] = 0.0;
+                if constexpr (D > 0) {
+                    double x[DQ];
+                    // unconditional (invalid lanes read position 0, masked below)
+#pragma unroll
+                    for (int l = 0; l < DQ; l++) x[l] = soa[(uint64_t)l * n + pos];
+#pragma unroll
+                    for (int qi = 0; qi < kQB; qi++) {
+#pragma unroll
+                        for (int l = 0; l < DQ; l++) {
+                            const double d = x[l] - qreg[qi][l];
+                            acc[qi] = __builtin_fma(d, d, acc[qi]);
+                        }
+                    }
+                } else {
+                    for (uint32_t l = 0; l < dim; l++) {
+                        const double xl = valid ? soa[(uint64_t)l * n + pos] : 0.0;
+#pragma unroll
+                        for (int qi = 0; qi < kQB; qi++) {
+                            const double d = xl - a.queries[(uint64_t)qid[qi] * dim + l];
+                            acc[qi] = __builtin_fma(d, d, acc[qi]);
+                        }
+                    }
+                }
+                uint32_t pid = 0;
+                if (FILL && valid) pid = a.ix.ila[pos];
+#pragma unroll
+                for (int qi = 0; qi < kQB; qi++) {
+                    const bool hit = valid && (uint32_t)qi < nqg && acc[qi] <= T;
+                    const uint64_t m = __ballot(hit);
+                    if (FILL && hit) {
+                        a.out[base[qi] + cnt[qi] + lanes_below(m)] = pid;
+                        my_hash += mix64(((uint64_t)qid[qi] << 32) | pid);
+                    }
+                    cnt[qi] += (uint32_t)__popcll(m);
+                }
+            }
+            if (!FILL && lane < (int)nqg) {
+                uint32_t mine = 0;
+#pragma unroll
+                for (int qi = 0; qi < kQB; qi++)
+                    if (qi == lane) mine = cnt[qi];
+                a.counts[qid[lane]] = mine;
+            }
+            if (lane == 0) my_evals += (uint64_t)M * nqg;
+        }
+        wave_lds_sync();  // the row list is rebuilt for the next unit
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) my_hash += __shfl_xor(my_hash, o, 64);
+    if (lane == 0) {
+        if (FILL && my_hash) atomicAdd(a.hash, (unsigned long long)my_hash);
+        if (!FILL && my_evals) atomicAdd(a.evals, (unsigned long long)my_evals);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_q_u32_u64(const uint32_t *__restrict__ in, uint32_t n,
+                                                   uint64_t *__restrict__ out) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = in[i];
+}
+
+template <int D, bool FILL>
+hegj_status launch_q(const QJoinArgs &a, uint32_t blocks, hipStream_t s) {
+    k_qjoin<D, FILL><<<blocks, 64 * kQWaves, 0, s>>>(a);
+    HIP_TRY(hipGetLastError());
+    return HEGJ_OK;
+}
+
+template <bool FILL>
+hegj_status launch_qd(const QJoinArgs &a, uint32_t blocks, hipStream_t s) {
+    switch (a.qg.g.dim) {
+        case 2: return launch_q<2, FILL>(a, blocks, s);
+        case 6: return launch_q<6, FILL>(a, blocks, s);
+        default: return launch_q<0, FILL>(a, blocks, s);
+    }
+}
+
+}  // namespace
+
+hegj_status query_geom(const GridGeom &g, QueryGeom *qg) {
+    qg->g = g;
+    uint64_t prod = 1;
+    for (uint32_t j = 0; j < g.k; j++) {
+        const uint64_t w = (uint64_t)g.nCells[j] + 2;
+        if (prod > (~0ull >> 1) / w) return fail(HEGJ_ERR_UNSUPPORTED, "query grid needs more than 63 bits");
+        prod *= w;
+    }
+    qg->dead_key = prod;  // one past the largest live key
+    int bits = 0;
+    while (bits < 64 && (prod >> bits) != 0) bits++;
+    qg->key_bits = bits;
+    return HEGJ_OK;
+}
+
+hegj_status query_plan(QueryPlan *p, Scratch &keep, const QueryGeom &qg, const double *queries,
+                       uint32_t nq, hipStream_t s) {
+    p->nq = nq;
+    p->n_units = 0;
+    p->n_live = 0;
+    if (nq == 0) return HEGJ_OK;
+    HIP_TRY(keep.get(&p->keys, nq));
+    HIP_TRY(keep.get(&p->ids, nq));
+    HIP_TRY(keep.get(&p->starts, (size_t)nq + 1));
+    Scratch sc;
+    uint64_t *keys_alt;
+    uint32_t *ids_alt, *flag, *rank, *d_small;
+    HIP_TRY(sc.get(&keys_alt, nq));
+    HIP_TRY(sc.get(&ids_alt, nq));
+    HIP_TRY(sc.get(&flag, nq));
+    HIP_TRY(sc.get(&rank, nq));
+    HIP_TRY(sc.get(&d_small, 2));
+    k_qcell_keys<<<ceil_div(nq, 256), 256, 0, s>>>(queries, nq, qg, p->keys, p->ids);
+    HIP_TRY(hipGetLastError());
+    HEGJ_TRY(radix_sort_u64(p->keys, p->ids, keys_alt, ids_alt, nq, qg.key_bits, s));
+    HIP_TRY(hipMemsetAsync(d_small, 0, 2 * sizeof(uint32_t), s));
+    k_qrun_flags<<<ceil_div(nq, 256), 256, 0, s>>>(p->keys, nq, qg.dead_key, flag, d_small + 1);
+    HIP_TRY(hipGetLastError());
+    HEGJ_TRY(scan_u32(flag, rank, nq, d_small, s));
+    k_qrun_fill<<<ceil_div(nq, 256), 256, 0, s>>>(flag, rank, nq, p->starts);
+    HIP_TRY(hipGetLastError());
+    uint32_t h[2];
+    HIP_TRY(hipMemcpyAsync(h, d_small, sizeof(h), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    p->n_units = h[0];
+    p->n_live = h[1];
+    // starts[n_units] = end of the live queries
+    HIP_TRY(hipMemcpyAsync(p->starts + p->n_units, &p->n_live, sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return HEGJ_OK;
+}
+
+hegj_status query_run(const QJoinArgs &base, bool fill, int device, hipStream_t s) {
+    if (base.n_units == 0) return HEGJ_OK;
+    // resident workgroups of the join on this device (thread-safe, once)
+    static const uint32_t resident = [device]() {
+        hipDeviceProp_t prop;
+        int per_cu = 0;
+        if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 1024u;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, reinterpret_cast<const void *>(&k_qjoin<6, true>), 64 * kQWaves, 0) != hipSuccess)
+            per_cu = 1;
+        return (uint32_t)std::max(per_cu, 1) * (uint32_t)prop.multiProcessorCount;
+    }();
+    const uint32_t blocks =
+        std::max<uint32_t>(1, std::min<uint32_t>(resident, ceil_div(base.n_units, kQWaves)));
+    return fill ? launch_qd<true>(base, blocks, s) : launch_qd<false>(base, blocks, s);
+}
+
+hegj_status query_offsets(const uint32_t *counts, uint32_t nq, uint64_t *offsets, hipStream_t s) {
+    Scratch sc;
+    uint64_t *c64;
+    HIP_TRY(sc.get(&c64, nq));
+    k_q_u32_u64<<<ceil_div(nq, 256), 256, 0, s>>>(counts, nq, c64);
+    HIP_TRY(hipGetLastError());
+    return scan_u64(c64, offsets, nq, offsets + nq, s);
+}
+
+}  // namespace hegj

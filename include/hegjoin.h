@@ -201,7 +201,7 @@ typedef struct {
     double total_ms;           /* host wall time of hegj_selfjoin */
     double join_kernel_avg_ms; /* average duration of one join kernel launch */
     uint32_t join_launches;
-    uint32_t reserved;
+    uint32_t refine_used;      /* hegj_refine that ran: FP64 or MFMA */
 } hegj_result_info;
 hegj_status hegj_result_get_info(const hegj_result *res, hegj_result_info *info);
 
@@ -212,6 +212,80 @@ hegj_status hegj_result_copy_counts(const hegj_result *res, uint32_t *counts);
  * sorted ascending when sort_rows != 0. */
 hegj_status hegj_result_copy_csr(const hegj_result *res, uint64_t *offsets, uint32_t *neighbors,
                                  int sort_rows);
+
+/* ------------------------------------------------------------------------
+ * Serving: a device-resident index answering range-join requests.
+ *
+ * No counterpart in the reference (its only join is the self-join of
+ * main.cu:255-283).  A request is a set of query points and an epsilon; the
+ * answer is, for each query, every index point p with sqrt(acc) <= eps,
+ * acc = fma(p_l - q_l, p_l - q_l, acc) in dimension order (the reference's
+ * chain, kernel.cu:350-355), as CSR rows of original point ids.
+ *
+ * The index is kept at several grid levels (cell sides); a request runs on
+ * the smallest cell side c with eps <= c * (1 - 2^-30).  `slots` requests are
+ * on the device at once, each on its own stream: requests that arrive while
+ * others run start as soon as a slot frees up (continuous batching).  Each
+ * request first counts its result exactly, then waits (first come, first
+ * served) until the result-buffer pool of `result_bytes` has room for it;
+ * results stay in the pool until released.  A request whose result alone is
+ * larger than the pool fails with HEGJ_ERR_CAPACITY.
+ * ------------------------------------------------------------------------ */
+#define HEGJ_SERVE_MAX_LEVELS 8
+typedef struct hegj_server hegj_server;
+
+typedef struct {
+    uint32_t dim;              /* point dimensionality */
+    uint32_t indexed_dims;     /* grid dimensions k (1..6) */
+    uint32_t n_levels;         /* grid levels, 1..HEGJ_SERVE_MAX_LEVELS */
+    uint32_t slots;            /* requests on the device at once, 1..16 */
+    double cell_sides[HEGJ_SERVE_MAX_LEVELS]; /* cell side of each level */
+    uint64_t result_bytes;     /* result-buffer pool (admission budget) */
+    uint32_t max_queries;      /* largest request */
+    int32_t device;
+} hegj_serve_opts;
+
+typedef struct {
+    uint64_t ticket;
+    int32_t status;            /* hegj_status of the request */
+    uint32_t level;            /* grid level used */
+    uint32_t n_queries;
+    uint32_t reserved;
+    uint64_t pairs;            /* result size */
+    uint64_t pair_hash;        /* sum of mix64(query<<32 | point) over pairs */
+    uint64_t evals;            /* distance evaluations */
+    uint64_t result_bytes;     /* pool bytes held by the result */
+    double queue_ms;           /* submit -> on a slot */
+    double count_ms;           /* on a slot -> result size known */
+    double admit_ms;           /* waiting for pool memory */
+    double fill_ms;            /* admitted -> result written */
+    double latency_ms;         /* submit -> result ready */
+} hegj_request_info;
+
+typedef struct {
+    uint64_t submitted, completed, failed;
+    uint64_t pairs;            /* over completed requests */
+    double p50_ms, p99_ms, max_ms, mean_ms; /* latency over completed requests */
+    uint64_t pool_bytes, pool_in_use, pool_peak;
+    double wall_ms;            /* first submit -> last completion */
+} hegj_serve_stats;
+
+/* Upload `database` (n x dim, row-major) once and build every level. */
+hegj_status hegj_serve_open(const double *database, uint32_t n, const hegj_serve_opts *opts,
+                            hegj_server **out);
+/* Queue a request; `queries` (nq x dim) is copied before the call returns. */
+hegj_status hegj_serve_submit(hegj_server *srv, const double *queries, uint32_t nq, double epsilon,
+                              uint64_t *ticket);
+/* Block until the request has finished (or failed); fills *info. */
+hegj_status hegj_serve_wait(hegj_server *srv, uint64_t ticket, hegj_request_info *info);
+/* Copy a finished request's CSR: offsets[nq+1], neighbors[pairs]. */
+hegj_status hegj_serve_copy(hegj_server *srv, uint64_t ticket, uint64_t *offsets, uint32_t *neighbors);
+/* Return the request's pool memory and forget it. */
+hegj_status hegj_serve_release(hegj_server *srv, uint64_t ticket);
+/* Latency percentiles and pool use; reset != 0 starts a new window. */
+hegj_status hegj_serve_get_stats(hegj_server *srv, hegj_serve_stats *stats, int reset);
+/* Finish queued work, stop the slots and free everything. */
+hegj_status hegj_serve_close(hegj_server *srv);
 
 #ifdef __cplusplus
 }

@@ -668,6 +668,43 @@ void og_brute_fill(const double *pts, uint32_t n, uint32_t dim, uint32_t k, doub
     }
 }
 
+// Range join of external queries against a point set (the serving path; no
+// reference counterpart): every p with sqrt(acc) <= eps, acc the fma chain of
+// kernel.cu:350-355 over (p - q), neighbours in ascending id order.
+uint64_t og_range_counts(const double *pts, uint32_t n, const double *qs, uint32_t nq, uint32_t dim,
+                         double eps, uint32_t *counts) {
+    uint64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : total)
+    for (int64_t q = 0; q < (int64_t)nq; q++) {
+        const double *xq = qs + (size_t)q * dim;
+        uint32_t cnt = 0;
+        for (uint32_t c = 0; c < n; c++) {
+            const double *xc = pts + (size_t)c * dim;
+            double acc = 0;
+            for (uint32_t l = 0; l < dim; l++) acc = std::fma(xc[l] - xq[l], xc[l] - xq[l], acc);
+            cnt += std::sqrt(acc) <= eps;
+        }
+        counts[q] = cnt;
+        total += cnt;
+    }
+    return total;
+}
+
+void og_range_fill(const double *pts, uint32_t n, const double *qs, uint32_t nq, uint32_t dim,
+                   double eps, const uint64_t *offsets, uint32_t *nbrs) {
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t q = 0; q < (int64_t)nq; q++) {
+        const double *xq = qs + (size_t)q * dim;
+        uint64_t o = offsets[q];
+        for (uint32_t c = 0; c < n; c++) {
+            const double *xc = pts + (size_t)c * dim;
+            double acc = 0;
+            for (uint32_t l = 0; l < dim; l++) acc = std::fma(xc[l] - xq[l], xc[l] - xq[l], acc);
+            if (std::sqrt(acc) <= eps) nbrs[o++] = c;
+        }
+    }
+}
+
 uint64_t og_pair_mix(uint32_t key, uint32_t value) {
     return mix64(((uint64_t)key << 32) | (uint64_t)value);
 }

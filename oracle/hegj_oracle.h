@@ -111,6 +111,10 @@ void og_join_fill(const double *pts, uint32_t n, uint32_t dim, uint32_t k, doubl
  * computed literally with sqrt; a query whose cell has a zero coordinate in
  * any of the first k dims emits nothing when emulate_cell0.  Same outputs as
  * og_join_counts / og_join_fill. */
+uint64_t og_range_counts(const double *pts, uint32_t n, const double *qs, uint32_t nq, uint32_t dim,
+                         double eps, uint32_t *counts);
+void og_range_fill(const double *pts, uint32_t n, const double *qs, uint32_t nq, uint32_t dim,
+                   double eps, const uint64_t *offsets, uint32_t *nbrs);
 uint64_t og_brute_counts(const double *pts, uint32_t n, uint32_t dim, uint32_t k, double eps,
                          int emulate_cell0, uint32_t *counts);
 void og_brute_fill(const double *pts, uint32_t n, uint32_t dim, uint32_t k, double eps,

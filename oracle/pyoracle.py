@@ -63,6 +63,9 @@ def lib():
         L.og_brute_counts.argtypes = [_f64p, u32, u32, u32, f64, i32, _u32p]
         L.og_brute_counts.restype = u64
         L.og_brute_fill.argtypes = [_f64p, u32, u32, u32, f64, i32, _u64p, _u32p]
+        L.og_range_counts.argtypes = [_f64p, u32, _f64p, u32, u32, f64, _u32p]
+        L.og_range_counts.restype = u64
+        L.og_range_fill.argtypes = [_f64p, u32, _f64p, u32, u32, f64, _u64p, _u32p]
         L.og_count_cell0.argtypes = [_f64p, u32, u32, u32, f64]
         L.og_count_cell0.restype = u32
         L.og_pair_mix.argtypes = [u32, u32]
@@ -183,6 +186,18 @@ def brute(pts, k, eps, emulate_cell0=True):
     L = lib()
     return _csr(lambda c: L.og_brute_counts(flat, n, d, k, eps, int(emulate_cell0), c),
                 lambda o, b: L.og_brute_fill(flat, n, d, k, eps, int(emulate_cell0), o, b), n)
+
+
+def range_join(pts, queries, eps):
+    """Brute-force range join of `queries` against `pts` (serving oracle):
+    CSR over queries, neighbour ids ascending."""
+    pts, n, d = _pts(pts)
+    qs, nq, dq = _pts(queries)
+    assert dq == d
+    fp, fq = pts.reshape(-1), qs.reshape(-1)
+    L = lib()
+    return _csr(lambda c: L.og_range_counts(fp, n, fq, nq, d, eps, c),
+                lambda o, b: L.og_range_fill(fp, n, fq, nq, d, eps, o, b), nq)
 
 
 def count_cell0(pts, k, eps):

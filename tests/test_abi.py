@@ -44,6 +44,85 @@ def test_struct_layouts_match_reference():
     assert H.SCHED_DTYPE.itemsize == 8         # struct schedulingCell
 
 
+PROBE = r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "hegjoin.h"
+#define S(T) printf(#T " %zu\n", sizeof(T));
+#define O(T, f) printf(#T "." #f " %zu\n", offsetof(T, f));
+int main(void) {
+    S(hegj_index_opts) S(hegj_index_info) S(hegj_join_opts) S(hegj_result_info)
+    S(hegj_serve_opts) S(hegj_request_info) S(hegj_serve_stats)
+    O(hegj_index_info, build_ms) O(hegj_join_opts, part_by_block)
+    O(hegj_result_info, refine_used) O(hegj_serve_opts, result_bytes)
+    O(hegj_serve_opts, device) O(hegj_request_info, latency_ms) O(hegj_serve_stats, wall_ms)
+    return 0;
+}
+"""
+
+
+def test_ctypes_mirrors_match_c_layouts(tmp_path):
+    """Every ctypes structure of the Python mirror has the C header's size and
+    field offsets (compiled probe)."""
+    import hegjoin_amd as H
+    src = tmp_path / "probe.c"
+    src.write_text(PROBE)
+    exe = str(tmp_path / "probe")
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", exe], check=True)
+    got = dict(line.rsplit(" ", 1) for line in subprocess.run(
+        [exe], capture_output=True, text=True, check=True).stdout.splitlines())
+    mirror = {"hegj_index_opts": H._IndexOpts, "hegj_index_info": H._IndexInfo,
+              "hegj_join_opts": H._JoinOpts, "hegj_result_info": H._ResultInfo,
+              "hegj_serve_opts": H._ServeOpts, "hegj_request_info": H._RequestInfo,
+              "hegj_serve_stats": H._ServeStats}
+    for name, cls in mirror.items():
+        assert int(got[name]) == C.sizeof(cls), name
+    for key, val in got.items():
+        if "." in key:
+            name, field = key.split(".")
+            assert getattr(mirror[name], field).offset == int(val), key
+
+
+def test_info_structs_convert_to_dicts():
+    """Python mirror: scalar fields stay scalars, arrays become lists, the
+    refine code becomes its name."""
+    import hegjoin_amd as H
+    r = H._ResultInfo()
+    r.refine_used, r.total_neighbors = 2, 7
+    d = H._struct_dict(r)
+    assert d["refine_used"] == "mfma" and d["total_neighbors"] == 7 and "reserved" not in d
+    ii = H._IndexInfo()
+    ii.nCells[0] = 5
+    d = H._struct_dict(ii)
+    assert isinstance(d["nCells"], list) and d["nCells"][0] == 5
+
+
+def test_serve_argument_validation():
+    import hegjoin_amd as H
+    lib = H.library()
+    pts = np.random.rand(100, 3)
+    o = H._ServeOpts()
+    h = C.c_void_p()
+
+    def opts(**kw):
+        o.dim, o.indexed_dims, o.n_levels, o.slots = 3, 3, 2, 2
+        o.cell_sides[0], o.cell_sides[1] = 0.05, 0.1
+        o.result_bytes, o.max_queries, o.device = 1 << 20, 1000, 0
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return C.byref(o)
+    bad = [dict(indexed_dims=4), dict(n_levels=0), dict(n_levels=9), dict(slots=0),
+           dict(slots=17), dict(max_queries=0), dict(result_bytes=100)]
+    for kw in bad:
+        assert lib.hegj_serve_open(pts.ctypes.data, 100, opts(**kw), C.byref(h)) == 1, kw
+    o2 = opts()
+    o.cell_sides[1] = 0.05  # not ascending
+    assert lib.hegj_serve_open(pts.ctypes.data, 100, o2, C.byref(h)) == 1
+    assert lib.hegj_serve_open(None, 100, opts(), C.byref(h)) == 1
+    assert lib.hegj_serve_close(None) == 0
+    assert lib.hegj_serve_submit(None, None, 0, 0.1, None) == 1
+
+
 def test_version_and_no_device_status():
     import hegjoin_amd as H
     assert "gfx950" in H.version()

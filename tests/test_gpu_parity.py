@@ -117,10 +117,78 @@ def test_join_random_vs_oracle(H, oracle, dim, k, n, eps, seed):
     k < dim, against the oracle computed live."""
     pts = oracle.generate_expo(n, dim, seed=seed)
     print("case dim", dim, "k", k, "n", n, flush=True)
-    off, nb, info = gpu_join(H, pts, eps, k)
+    off, nb, info = gpu_join(H, pts, eps, k, refine="fp64")
     eoff, enb = oracle.join(pts, k, eps)
     assert np.array_equal(off, eoff) and np.array_equal(nb, enb)
     assert info["pair_hash"] == oracle.csr_hash(eoff, enb)
+
+
+HIGH = golden_files("alg0_d18") + golden_files("alg0_d90")
+
+
+@pytest.mark.parametrize("refine", ["fp64", "mfma"])
+@pytest.mark.parametrize("path", HIGH, ids=os.path.basename)
+def test_high_dim_golden_both_refines(H, path, refine):
+    """18-D and 90-D fixtures through the plain FP64 kernel and through the
+    split-BF16 MFMA guard band: both must give the fixture's pair set."""
+    g = load_golden(path)
+    off, nb, info = gpu_join(H, g["pts"], float(g["eps"]), int(g["k"]), refine=refine)
+    assert info["refine_used"] == refine
+    assert np.array_equal(off, g["offsets"]) and np.array_equal(nb, g["nbrs"])
+    assert info["pair_hash"] == int(g["hash"])
+
+
+@pytest.mark.parametrize("dim,k,n,eps,seed", [
+    (16, 6, 3000, 0.12, 21), (18, 6, 4000, 0.15, 22), (20, 4, 3000, 0.15, 23),
+    (29, 6, 2000, 0.25, 24), (40, 6, 1500, 0.3, 25), (64, 3, 1200, 0.4, 26),
+    (90, 6, 1500, 0.45, 27), (100, 5, 1000, 0.5, 28), (120, 2, 800, 0.5, 29)])
+def test_mfma_refine_vs_oracle(H, oracle, dim, k, n, eps, seed):
+    """Every instantiated K-block count of the MFMA kernel (K = 3(dim+2)
+    padded to 64 .. 384) against the FP64 oracle."""
+    pts = oracle.generate_expo(n, dim, seed=seed)
+    print("case mfma dim", dim, "k", k, "n", n, flush=True)
+    off, nb, info = gpu_join(H, pts, eps, k, refine="mfma")
+    eoff, enb = oracle.join(pts, k, eps)
+    assert info["refine_used"] == "mfma"
+    assert np.array_equal(off, eoff) and np.array_equal(nb, enb)
+    assert info["pair_hash"] == oracle.csr_hash(eoff, enb)
+
+
+def planted_boundary(dim, n_base, eps, seed):
+    """Random points plus, for each, partners at distance eps (1 + r) with
+    |r| <= 1e-9 along random directions: pairs whose FP64 distance sits on
+    the eps boundary, decided only by the FP64 recheck."""
+    rng = np.random.default_rng(seed)
+    base = 0.3 + 0.4 * rng.random((n_base, dim))
+    out = [base]
+    for rel in (-1e-9, -1e-15, 0.0, 1e-15, 1e-9):
+        d = rng.standard_normal((n_base, dim))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        out.append(base + d * eps * (1 + rel))
+    return np.ascontiguousarray(np.concatenate(out))
+
+
+@pytest.mark.parametrize("dim,k", [(18, 6), (90, 6), (33, 3)])
+def test_mfma_boundary_pairs_rechecked(H, oracle, dim, k):
+    """Pairs on the eps boundary land in the guard band; the FP64 recheck
+    must decide each exactly as the oracle does (sqrt(acc) <= eps)."""
+    eps = 0.05
+    pts = planted_boundary(dim, 400, eps, seed=dim)
+    off, nb, info = gpu_join(H, pts, eps, k, refine="mfma")
+    eoff, enb = oracle.join(pts, k, eps)
+    assert info["band_rechecks"] > 0
+    assert np.array_equal(off, eoff) and np.array_equal(nb, enb)
+
+
+def test_mfma_large_offset_coordinates(H, oracle):
+    """Coordinates far from the origin (centring keeps the BF16 split
+    accurate) and duplicates (distance 0)."""
+    rng = np.random.default_rng(4)
+    pts = 1000.0 + rng.random((2000, 24))
+    pts[1000:1100] = pts[:100]
+    off, nb, info = gpu_join(H, pts, 0.6, 6, refine="mfma")
+    eoff, enb = oracle.join(pts, 6, 0.6)
+    assert np.array_equal(off, eoff) and np.array_equal(nb, enb)
 
 
 @pytest.mark.parametrize("n,eps", [(40000, 0.004), (60000, 0.005)])

@@ -325,3 +325,43 @@ def test_alg3_vs_alg0_differences_explained(oracle, path):
         for c in diff:
             d = np.sqrt(((pts[c] - pts[q]) ** 2).sum())
             assert abs(d - eps) <= 1e-12, (q, c, d, eps)
+
+
+# ---- serving oracle: brute-force range join --------------------------------
+
+def test_range_join_of_the_points_themselves_is_the_selfjoin(oracle):
+    """Queries = the indexed points: the range join is the brute self-join
+    without the cell-0 wrap (both brute force, written independently)."""
+    pts = oracle.generate_expo(1500, 3, seed=41)
+    off, nb = oracle.range_join(pts, pts, 0.02)
+    eoff, enb = oracle.brute(pts, 3, 0.02, emulate_cell0=False)
+    assert np.array_equal(off, eoff) and np.array_equal(nb, enb)
+
+
+def test_range_join_lattice_closed_form(oracle):
+    """Integer lattice (spacing 1/8, exact in binary) and queries on lattice
+    points: with eps = 1/8 exactly the neighbours are the query's lattice
+    point and its 2*dim axis neighbours inside the box."""
+    g = np.arange(8) / 8.0
+    pts = np.array(np.meshgrid(g, g, g, indexing="ij")).reshape(3, -1).T.copy()
+    qs = pts[[0, 73, 511]]
+    off, nb = oracle.range_join(pts, qs, 0.125)
+    expect = []
+    for q in qs:
+        d = np.abs(pts - q).sum(1)
+        expect.append(np.flatnonzero((d == 0) | ((d == 0.125) & ((pts != q).sum(1) == 1))))
+    assert np.array_equal(np.diff(off), [len(e) for e in expect])
+    assert np.array_equal(nb, np.concatenate(expect))
+
+
+def test_range_join_boundary_is_sqrt_le_eps(oracle):
+    """A pair at exactly distance eps (3-4-5 triangle scaled by a power of
+    two) is included; the next representable larger offset is not."""
+    pts = np.array([[0.0, 0.0], [0.375, 0.5]])
+    q = np.array([[0.0, 0.0]])
+    off, nb = oracle.range_join(pts, q, 0.625)
+    assert list(nb) == [0, 1]
+    pts2 = pts.copy()
+    pts2[1, 1] = np.nextafter(0.5, 1.0)
+    off, nb = oracle.range_join(pts2, q, 0.625)
+    assert list(nb) == [0]
