@@ -91,6 +91,39 @@ def cpu_baseline(pts, eps, dim, seconds_budget=20.0):
             "kind": "port", "sample": f"Super-EGO restatement, {len(qids)} queries"}
 
 
+def rooflines(n, dim, cells, pairs, evals, kernel_ms, refine):
+    """The dominant join kernel against its two roofs, per launch.
+
+    Algorithmic work (per unit, times the units one launch processes):
+      HBM bytes   = 8 n D (coordinates once) + 4 n (indexLookupArr)
+                    + 24 C (grid + gridCellLookup per cell) + 8 P (query and
+                    neighbour id per pair)
+      FP64 flops  = 3 D per reference candidate E (subtract + fma per dim)
+      MFMA flops  = 2 D per candidate pair, against the bf16 peak divided by
+                    the 3 products of a split-BF16 dot (hi.hi + hi.lo + lo.hi)
+    Returns (binding, other): the binding roof is the one whose roof time is
+    longer (the roof the kernel would hit first)."""
+    t = kernel_ms * 1e-3
+    bytes_alg = 8.0 * n * dim + 4.0 * n + 24.0 * cells + 8.0 * pairs
+    hbm = {"bound": "hbm", "achieved": bytes_alg / t / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+           "frac": bytes_alg / t / 1e9 / PEAK_HBM_GBS, "traffic": None,
+           "work": bytes_alg, "roof_ms": bytes_alg / (PEAK_HBM_GBS * 1e9) * 1e3,
+           "note": "8nD + 4n + 24C + 8P bytes per launch"}
+    if refine == "mfma":
+        flops = 2.0 * dim * evals
+        peak = PEAK_BF16_TFLOPS / 3.0
+        note = ("2 D flops per candidate pair; peak = bf16 dense MFMA / 3 (split-BF16 "
+                "hi.hi + hi.lo + lo.hi); kernel k_gb_join")
+    else:
+        flops = 3.0 * dim * evals
+        peak = PEAK_FP64_TFLOPS
+        note = "3 D FP64 flops (sub + fma per dim) per reference candidate; kernel k_join"
+    comp = {"bound": "mfma", "achieved": flops / t / 1e12, "peak": peak, "unit": "TFLOP/s",
+            "frac": flops / t / 1e12 / peak, "traffic": None, "work": flops,
+            "roof_ms": flops / (peak * 1e12) * 1e3, "note": note}
+    return (comp, hbm) if comp["roof_ms"] >= hbm["roof_ms"] else (hbm, comp)
+
+
 def serve_levels(eps_lo, eps_hi):
     """Grid levels for the serving index: cell sides from just above eps_hi
     down by factors of sqrt(2) while they still exceed eps_lo / sqrt(2)."""
@@ -196,12 +229,14 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--calibrate", action="store_true")
-    ap.add_argument("--queries-per-unit", type=int, default=0)
-    ap.add_argument("--unit-order", default="morton")
-    ap.add_argument("--xcd-parts", type=int, default=0)
+    ap.add_argument("--queries-per-unit", type=int, default=None)
+    ap.add_argument("--unit-order", default=None, choices=["spatial", "workload", "morton"])
+    ap.add_argument("--xcd-parts", type=int, default=None)
     ap.add_argument("--parts-by-block", type=int, default=0)
-    ap.add_argument("--grab", type=int, default=0)
-    ap.add_argument("--refine", default="auto", choices=["auto", "fp64", "mfma"])
+    ap.add_argument("--grab", type=int, default=None)
+    ap.add_argument("--refine", default=None, choices=["auto", "fp64", "mfma"])
+    ap.add_argument("--graph", action="store_true",
+                    help="replay the captured batch graph (FP64 refine)")
     ap.add_argument("--n", type=int, default=0, help="override the point count (experiments)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -233,9 +268,12 @@ def main():
     cells, _, total_cand = ix.sort_by_workload()
     print(f"# index: {iinfo['nNonEmptyCells']} cells, build {iinfo['build_ms']:.1f} ms, "
           f"candidates {total_cand}", file=sys.stderr, flush=True)
-    kw = dict(queries_per_unit=a.queries_per_unit, unit_order=a.unit_order,
-              xcd_parts=a.xcd_parts, units_per_grab=a.grab, part_by_block=a.parts_by_block,
-              refine=a.refine)
+    # options not given on the command line take the tuned defaults (tuning.tsv)
+    kw = dict(part_by_block=a.parts_by_block, use_graph=a.graph)
+    for key, val in (("queries_per_unit", a.queries_per_unit), ("unit_order", a.unit_order),
+                     ("xcd_parts", a.xcd_parts), ("units_per_grab", a.grab), ("refine", a.refine)):
+        if val is not None:
+            kw[key] = val
     for _ in range(a.warmup):
         r = ix.selfjoin(**kw)
         r.close()
@@ -255,19 +293,10 @@ def main():
     launches = sum(i["join_launches"] for i in infos)
     kernel_ms = sum(i["join_ms"] for i in infos) / launches
     refine_used = infos[-1]["refine_used"]
-    if refine_used == "mfma":
-        # split-BF16 Gram tile: K = 3 (dim + 2) products per candidate pair, 2 flops each
-        flops_alg = 2.0 * 3 * (dim + 2) * total_cand / (launches / a.steps)
-        peak_tf, roof_note = PEAK_BF16_TFLOPS, (
-            "MFMA bf16 dense: 2 * 3(dim+2) flops per reference candidate pair (split-BF16 "
-            "hi/lo Gram product), kernel k_gb_join")
-    else:
-        flops_alg = 3.0 * dim * total_cand / (launches / a.steps)  # sub + fma per dim per eval
-        peak_tf, roof_note = PEAK_FP64_TFLOPS, (
-            "FP64 compute bound: 3*dim flops (sub + fma per dim) per reference candidate; "
-            "FP64 vector and matrix peaks are equal, kernel k_join")
-    achieved_tf = flops_alg / (kernel_ms * 1e-3) / 1e12
-    bytes_alg = 8.0 * n * dim + 4.0 * n + 24.0 * iinfo["nNonEmptyCells"] + 4.0 * pairs + 12.0 * n
+    per_launch = launches / a.steps
+    roof, roof_alt = rooflines(n, dim, iinfo["nNonEmptyCells"], pairs, total_cand / per_launch,
+                               kernel_ms, refine_used)
+    roof_alt["attainable_copy_gbs"] = copy_gbs
     out = {
         "metric": "result pairs/s (1 GPU)",
         "value": pairs * a.steps / elapsed,
@@ -286,13 +315,8 @@ def main():
         "estimate_ms": infos[-1]["estimate_ms"],
         "refine": refine_used,
         "band_rechecks": infos[-1]["band_rechecks"],
-        "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": peak_tf,
-                     "unit": "TFLOP/s", "frac": achieved_tf / peak_tf, "traffic": None,
-                     "note": roof_note},
-        "roofline_hbm": {"bound": "hbm", "achieved": bytes_alg / (kernel_ms * 1e-3) / 1e9,
-                         "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                         "frac": bytes_alg / (kernel_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
-                         "attainable_copy": copy_gbs},
+        "roofline": roof,
+        "roofline_other": roof_alt,
     }
     ix.close()
     if not a.no_cpu_baseline:

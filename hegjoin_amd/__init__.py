@@ -63,7 +63,7 @@ class _JoinOpts(C.Structure):
     _fields_ = [("result_capacity", C.c_uint64), ("unit_order", C.c_uint32),
                 ("queries_per_unit", C.c_uint32), ("refine", C.c_uint32),
                 ("xcd_parts", C.c_uint32), ("units_per_grab", C.c_uint32),
-                ("part_by_block", C.c_uint32)]
+                ("part_by_block", C.c_uint32), ("use_graph", C.c_uint32)]
 
 
 class _ResultInfo(C.Structure):
@@ -73,7 +73,8 @@ class _ResultInfo(C.Structure):
                 ("n_batches", C.c_uint32), ("n_units", C.c_uint32), ("n_deferred", C.c_uint32),
                 ("estimate_ms", C.c_double), ("join_ms", C.c_double), ("workload_ms", C.c_double),
                 ("total_ms", C.c_double), ("join_kernel_avg_ms", C.c_double),
-                ("join_launches", C.c_uint32), ("refine_used", C.c_uint32)]
+                ("join_launches", C.c_uint32), ("refine_used", C.c_uint32),
+                ("graph_replay", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class _ServeOpts(C.Structure):
@@ -134,6 +135,9 @@ _lib.hegj_result_free.argtypes = [_vp]
 _lib.hegj_result_get_info.argtypes = [_vp, C.POINTER(_ResultInfo)]
 _lib.hegj_result_copy_counts.argtypes = [_vp, _vp]
 _lib.hegj_result_copy_csr.argtypes = [_vp, _vp, _vp, C.c_int]
+_lib.hegj_join_opts_tuned.argtypes = [C.POINTER(_JoinOpts), C.c_uint32]
+_lib.hegj_tuning_set.argtypes = [C.c_uint32, C.POINTER(_JoinOpts)]
+_lib.hegj_tuning_load.argtypes = [C.c_char_p, C.POINTER(C.c_uint32)]
 _lib.hegj_serve_open.argtypes = [_vp, C.c_uint32, C.POINTER(_ServeOpts), C.POINTER(_vp)]
 _lib.hegj_serve_submit.argtypes = [_vp, _vp, C.c_uint32, C.c_double, C.POINTER(C.c_uint64)]
 _lib.hegj_serve_wait.argtypes = [_vp, C.c_uint64, C.POINTER(_RequestInfo)]
@@ -146,6 +150,20 @@ _lib.hegj_serve_close.argtypes = [_vp]
 def _check(status):
     if status != 0:
         raise HegjError(status, _lib.hegj_last_error().decode())
+
+
+TUNING_PATH = os.path.join(_PKG, "tuning.tsv")
+
+
+def load_tuning(path=TUNING_PATH):
+    """Register a tuning table (dim qb order parts grab refine per line)."""
+    n = C.c_uint32()
+    _check(_lib.hegj_tuning_load(str(path).encode(), C.byref(n)))
+    return n.value
+
+
+if os.path.exists(TUNING_PATH):
+    load_tuning()
 
 
 def version() -> str:
@@ -274,21 +292,31 @@ class GridIndex:
                                           C.byref(tot)))
         return cells, opi, tot.value
 
-    def selfjoin(self, result_capacity=0, unit_order="morton", queries_per_unit=0, xcd_parts=0,
-                 refine="auto", units_per_grab=0, part_by_block=0):
-        """unit_order: "morton" (default), "spatial" (linear cell id) or
-        "workload" (descending candidates, the reference's order);
-        refine: "auto", "fp64" or "mfma" (split-BF16 MFMA + guard band +
-        FP64 band recheck); xcd_parts: unit ranges, one per XCD (0 = auto)."""
+    def selfjoin(self, result_capacity=0, unit_order=None, queries_per_unit=None, xcd_parts=None,
+                 refine=None, units_per_grab=None, part_by_block=0, use_graph=False):
+        """Options left at None take the tuned defaults for this
+        dimensionality (tuning.tsv, written by hegjoin_amd.autotune).
+        unit_order: "morton", "spatial" (linear cell id) or "workload"
+        (descending candidates, the reference's order); refine: "auto",
+        "fp64" or "mfma" (split-BF16 MFMA + guard band + FP64 band
+        recheck); xcd_parts: unit ranges, one per XCD (0 = auto);
+        use_graph: keep the FP64 plan in the index and replay a captured
+        hipGraph of the batch (repeated joins with the same options)."""
         o = _JoinOpts()
-        _lib.hegj_join_opts_default(C.byref(o))
+        _lib.hegj_join_opts_tuned(C.byref(o), self.dim)
         o.result_capacity = result_capacity
-        o.unit_order = {"spatial": 0, "workload": 1, "morton": 2}[unit_order]
-        o.queries_per_unit = queries_per_unit
-        o.xcd_parts = xcd_parts
-        o.refine = {"auto": 0, "fp64": 1, "mfma": 2}[refine]
-        o.units_per_grab = units_per_grab
+        if unit_order is not None:
+            o.unit_order = {"spatial": 0, "workload": 1, "morton": 2}[unit_order]
+        if queries_per_unit is not None:
+            o.queries_per_unit = queries_per_unit
+        if xcd_parts is not None:
+            o.xcd_parts = xcd_parts
+        if refine is not None:
+            o.refine = {"auto": 0, "fp64": 1, "mfma": 2}[refine]
+        if units_per_grab is not None:
+            o.units_per_grab = units_per_grab
         o.part_by_block = part_by_block
+        o.use_graph = 1 if use_graph else 0
         r = _vp()
         _check(_lib.hegj_selfjoin(self._h, C.byref(o), C.byref(r)))
         return NeighborTable(r, self)

@@ -1,0 +1,136 @@
+"""Autotuner of the join options, per dimensionality, on the GPU.
+
+    python -m hegjoin_amd.autotune --dim 6 --n 5000000 --eps 0.0063131373103199606
+    python -m hegjoin_amd.autotune --dim 18 --n 2000000 --eps 0.041121185939337571
+
+For one workload (the reference's exponential generator, seed 0) it sweeps
+the runtime choices of the join -- queries per work unit (the points that
+share one streamed candidate tile), unit order, per-XCD unit ranges, units
+per queue grab, and at 16 dimensions and above the refine (FP64 chain or the
+split-BF16 MFMA guard band) -- times each configuration, and checks that all
+of them produce the same pair count and pair hash.  A configuration whose
+result differs is an error, not a candidate.
+
+The fastest configuration (host wall time of hegj_selfjoin, mean over
+--steps runs) is written to tuning.tsv (one line per dimensionality: dim
+queries_per_unit unit_order xcd_parts units_per_grab refine), which the
+library, the Python package and the CLI load at start; every measurement goes
+to tuning.json.
+"""
+from __future__ import annotations
+
+import argparse
+import itertools
+import json
+import os
+import sys
+import time
+
+ORDERS = {"spatial": 0, "workload": 1, "morton": 2}
+REFINES = {"auto": 0, "fp64": 1, "mfma": 2}
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def candidates(dim):
+    """Option sets to try for one dimensionality."""
+    out = []
+    if dim >= 16:
+        for order in ("morton", "workload", "spatial"):
+            out.append(dict(refine="mfma", unit_order=order, queries_per_unit=0, xcd_parts=1,
+                            units_per_grab=0))
+        for qb in (4, 8):
+            out.append(dict(refine="fp64", unit_order="morton", queries_per_unit=qb, xcd_parts=8,
+                            units_per_grab=0))
+        return out
+    for qb, order, parts, grab in itertools.product((4, 8, 16), ("morton", "workload", "spatial"),
+                                                    (1, 8), (0, 1, 4)):
+        out.append(dict(refine="fp64", unit_order=order, queries_per_unit=qb, xcd_parts=parts,
+                        units_per_grab=grab))
+    return out
+
+
+def measure(ix, opts, steps, limit_s):
+    """Mean wall ms of hegj_selfjoin over `steps` runs (after one warm-up),
+    with the result's pair count and hash.  Stops early past limit_s."""
+    r = ix.selfjoin(**opts)
+    info = r.info
+    r.close()
+    if info["total_ms"] > limit_s * 1000:
+        return info["total_ms"], info, 1
+    times = []
+    for _ in range(steps):
+        r = ix.selfjoin(**opts)
+        info = r.info
+        r.close()
+        times.append(info["total_ms"])
+    return sum(times) / len(times), info, len(times)
+
+
+def tune(dim, n, eps, k=None, steps=2, limit_s=20.0, log=print):
+    import hegjoin_amd as H
+    pts = H.generate_expo(n, dim, seed=0)
+    k = k or min(dim, 6)
+    results = []
+    ref = None
+    with H.GridIndex(pts, eps, indexed_dims=k) as ix:
+        for opts in candidates(dim):
+            try:
+                ms, info, runs = measure(ix, opts, steps, limit_s)
+            except H.HegjError as e:
+                log(f"# skip {opts}: {e}")
+                continue
+            key = (info["total_neighbors"], info["pair_hash"])
+            if ref is None:
+                ref = key
+            if key != ref:
+                raise RuntimeError(f"configuration {opts} changed the result: {key} != {ref}")
+            row = dict(opts, ms=ms, runs=runs, kernel_ms=info["join_kernel_avg_ms"],
+                       pairs=info["total_neighbors"], pair_hash="%016x" % info["pair_hash"])
+            results.append(row)
+            log(json.dumps(row))
+    best = min(results, key=lambda r: r["ms"])
+    return best, results
+
+
+def write_tables(dim, workload, best, results, tsv=None, js=None):
+    tsv = tsv or os.path.join(HERE, "tuning.tsv")
+    js = js or os.path.join(HERE, "tuning.json")
+    table = {}
+    if os.path.exists(js):
+        with open(js) as f:
+            table = json.load(f)
+    table[str(dim)] = dict(workload=workload, best=best, results=results)
+    with open(js, "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+    lines = ["# dim queries_per_unit unit_order xcd_parts units_per_grab refine",
+             "# written by hegjoin_amd/autotune.py; workloads and timings in tuning.json"]
+    for d in sorted(table, key=int):
+        b = table[d]["best"]
+        lines.append(f"{d} {b['queries_per_unit']} {ORDERS[b['unit_order']]} {b['xcd_parts']} "
+                     f"{b['units_per_grab']} {REFINES[b['refine']]}")
+    with open(tsv, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return tsv, js
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--dim", type=int, required=True)
+    ap.add_argument("--n", type=int, required=True)
+    ap.add_argument("--eps", type=float, required=True)
+    ap.add_argument("--k", type=int, default=None)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--limit-s", type=float, default=20.0)
+    ap.add_argument("--tsv", default=None)
+    ap.add_argument("--json", default=None)
+    a = ap.parse_args(argv)
+    t0 = time.time()
+    best, results = tune(a.dim, a.n, a.eps, a.k, a.steps, a.limit_s,
+                         log=lambda s: print(s, file=sys.stderr, flush=True))
+    workload = dict(dim=a.dim, n=a.n, eps=a.eps, k=a.k or min(a.dim, 6), generator="expo seed 0")
+    tsv, js = write_tables(a.dim, workload, best, results, a.tsv, a.json)
+    print(json.dumps(dict(best=best, configs=len(results), seconds=time.time() - t0, tsv=tsv)))
+
+
+if __name__ == "__main__":
+    main()

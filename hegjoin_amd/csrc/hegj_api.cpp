@@ -8,8 +8,11 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <cstdio>
 #include <limits>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -79,6 +82,9 @@ struct EventTimer {
     }
 };
 
+struct GraphPlan;
+void free_graph_plan(GraphPlan *g);
+
 }  // namespace hegj
 
 using namespace hegj;
@@ -105,6 +111,8 @@ struct hegj_index {
     float *gb_w = nullptr;
     uint32_t gb_kp = 0;
     double gb_ms = 0;
+    // FP64 join plan + captured batch graph, reused while the options match
+    hegj::GraphPlan *gplan = nullptr;
 };
 
 struct hegj_result {
@@ -207,6 +215,321 @@ hegj_status ensure_morton(hegj_index *h) {
     if (h->d_morton) return HEGJ_OK;
     HIP_TRY(hipMalloc(&h->d_morton, sizeof(uint32_t) * (h->ix.C + 1)));
     return morton_order(h->g, h->ix, h->d_morton, &h->morton_ok, h->stream);
+}
+
+// Everything a FP64 join needs besides its result: work units in the chosen
+// order, their split into per-XCD ranges, the kernel's scratch and the batch
+// capacity (estimated on the device unless the caller fixed it).
+struct JoinPlan {
+    uint2 *units = nullptr, *units_next = nullptr;
+    uint32_t U = 0;
+    uint32_t *deferred = nullptr;
+    Counters *d_ctr = nullptr;
+    uint64_t *gmask = nullptr;
+    uint32_t gmask_slots = 8192;
+    uint32_t max_blocks = 0;
+    uint32_t parts = 1;
+    uint32_t part_end[kMaxParts] = {};
+    uint64_t capacity = 0;
+    double estimate_ms = 0;
+    JoinPlan() = default;
+    JoinPlan(const JoinPlan &) = delete;
+    JoinPlan &operator=(const JoinPlan &) = delete;
+    ~JoinPlan() {
+        for (void *p : {(void *)units, (void *)units_next, (void *)deferred, (void *)d_ctr, (void *)gmask})
+            if (p) (void)hipFree(p);
+    }
+};
+
+// About 32 grabs per wave keeps the queue atomics off the critical path.
+uint32_t grab_for(const JoinPlan &P, uint32_t nu) {
+    const uint64_t waves = (uint64_t)P.max_blocks * 4;
+    const uint64_t g = nu / (waves * 32);
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, 64));
+}
+
+uint32_t blocks_for(const JoinPlan &P, uint32_t nu, uint32_t grab) {
+    const uint64_t need = ((uint64_t)nu + grab * 4 - 1) / (grab * 4);
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(need, P.max_blocks));
+}
+
+void set_parts(JoinArgs &x, uint32_t parts, const uint32_t *ends) {
+    x.n_parts = parts;
+    for (uint32_t p = 0; p < parts; p++) {
+        x.part_begin[p] = p == 0 ? 0u : ends[p - 1];
+        x.part_end[p] = ends[p];
+    }
+}
+
+JoinArgs base_args(const hegj_index *h, const hegj_join_opts &opts, const JoinPlan &P) {
+    JoinArgs a{};
+    a.g = h->g;
+    a.ix = h->ix;
+    a.queue_heads = P.d_ctr->queue_heads;
+    a.n_deferred = &P.d_ctr->n_deferred;
+    a.cursor = &P.d_ctr->cursor;
+    a.hash = &P.d_ctr->hash;
+    a.evals = &P.d_ctr->evals;
+    a.pairs = &P.d_ctr->pairs;
+    a.deferred = P.deferred;
+    a.gmask = P.gmask;
+    a.gmask_slots = P.gmask_slots;
+    a.part_by_xcc = opts.part_by_block ? 0u : 1u;
+    return a;
+}
+
+hegj_status make_plan(hegj_index *h, const hegj_join_opts &opts, uint32_t qb, const uint32_t *order,
+                      JoinPlan &P) {
+    hipStream_t s = h->stream;
+    HEGJ_TRY(build_units(h->g, h->ix, qb, order, &P.units, &P.U, s));
+    const uint32_t U = P.U;
+    HIP_TRY(hipMalloc(&P.units_next, sizeof(uint2) * (U + 1)));
+    HIP_TRY(hipMalloc(&P.deferred, sizeof(uint32_t) * (U + 1)));
+    HIP_TRY(hipMalloc(&P.d_ctr, sizeof(Counters)));
+    HEGJ_TRY(join_occupancy(h->g.dim, qb, h->device, &P.max_blocks));
+    // per-wave hit-mask scratch for units whose masks exceed LDS
+    HIP_TRY(hipMalloc(&P.gmask, sizeof(uint64_t) * (size_t)P.max_blocks * 4 * P.gmask_slots));
+    // unit ranges per XCD, split by estimated work
+    P.parts = opts.xcd_parts ? std::min<uint32_t>(opts.xcd_parts, kMaxParts) : kMaxParts;
+    if (U < P.parts * 256) P.parts = 1;
+    HEGJ_TRY(split_units(h->ix, P.units, U, qb, h->d_work, P.parts, P.part_end, s));
+    // batch estimate: count-only pass over a strided sample of ~8192 units
+    // (a few per cent of the join's work at most)
+    P.capacity = opts.result_capacity;
+    if (P.capacity == 0 && U > 0) {
+        EventTimer t(s);
+        Scratch sc;
+        const uint32_t stride = std::max<uint32_t>(1, U / 8192);
+        const uint32_t ns = (U + stride - 1) / stride;
+        uint2 *sample;
+        HIP_TRY(sc.get(&sample, ns));
+        HEGJ_TRY(stride_units(P.units, ns, stride, sample, s));
+        HIP_TRY(hipMemsetAsync(P.d_ctr, 0, sizeof(Counters), s));
+        JoinArgs e = base_args(h, opts, P);
+        e.units = sample;
+        e.n_units = ns;
+        e.units_per_grab = grab_for(P, ns);
+        e.count_only = 1;
+        e.gmask_slots = 0;
+        uint32_t one_end = ns;
+        set_parts(e, 1, &one_end);
+        HEGJ_TRY(launch_join(e, qb, blocks_for(P, ns, e.units_per_grab), s));
+        Counters c;
+        HIP_TRY(hipMemcpyAsync(&c, P.d_ctr, sizeof(c), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        const double est = (double)c.pairs * ((double)U / (double)ns);
+        P.capacity = (uint64_t)(est * 1.10) + (1u << 20);
+        size_t free_b = 0, total_b = 0;
+        HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+        P.capacity = std::min<uint64_t>(P.capacity, (uint64_t)(free_b * 0.6) / sizeof(uint32_t));
+        P.estimate_ms = t.stop();
+    }
+    if (P.capacity == 0) P.capacity = 1;
+    return HEGJ_OK;
+}
+
+// Batches: each launch covers the pending units; a unit whose neighbours do
+// not fit in the batch's buffer is deferred to the next batch.
+hegj_status run_batches(hegj_index *h, const hegj_join_opts &opts, uint32_t qb, JoinPlan &P,
+                        hegj_result *r) {
+    hipStream_t s = h->stream;
+    const uint32_t n = h->g.n, U = P.U;
+    r->info.n_units = U;
+    r->info.estimate_ms = P.estimate_ms;
+    HIP_TRY(hipMalloc(&r->d_row_off, sizeof(uint64_t) * n));
+    HIP_TRY(hipMalloc(&r->d_row_cnt, sizeof(uint32_t) * n));
+    HIP_TRY(hipMemsetAsync(r->d_row_off, 0, sizeof(uint64_t) * n, s));
+    HIP_TRY(hipMemsetAsync(r->d_row_cnt, 0, sizeof(uint32_t) * n, s));
+    JoinArgs a = base_args(h, opts, P);
+    a.row_off = r->d_row_off;
+    a.row_cnt = r->d_row_cnt;
+    HIP_TRY(hipMemsetAsync(P.d_ctr, 0, sizeof(Counters), s));
+    const uint2 *cur = P.units;
+    uint32_t cur_n = U;
+    uint64_t base = 0, capacity = P.capacity;
+    double join_ms = 0;
+    uint32_t launches = 0;
+    Counters c{};
+    while (cur_n > 0) {
+        uint32_t *buf = nullptr;
+        HIP_TRY(hipMalloc(&buf, sizeof(uint32_t) * capacity));
+        r->batches.push_back(buf);
+        r->bases.push_back(base);
+        // reset per-batch counters (hash / evals / pairs accumulate)
+        HIP_TRY(hipMemsetAsync(P.d_ctr, 0, offsetof(Counters, hash), s));
+        a.units = cur;
+        a.n_units = cur_n;
+        a.units_per_grab = opts.units_per_grab ? opts.units_per_grab : grab_for(P, cur_n);
+        if (cur == P.units && cur_n == U) {
+            set_parts(a, P.parts, P.part_end);
+        } else {
+            uint32_t one_end = cur_n;
+            set_parts(a, 1, &one_end);
+        }
+        a.out = buf;
+        a.out_base = base;
+        a.capacity = capacity;
+        a.count_only = 0;
+        {
+            EventTimer t(s);
+            HEGJ_TRY(launch_join(a, qb, blocks_for(P, cur_n, a.units_per_grab), s));
+            join_ms += t.stop();
+            launches++;
+        }
+        HIP_TRY(hipMemcpyAsync(&c, P.d_ctr, sizeof(c), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (r->batches.size() >= (size_t)kMaxBatches && c.n_deferred)
+            return fail(HEGJ_ERR_CAPACITY, "result needs more than 64 batches");
+        base += capacity;
+        if (c.n_deferred == cur_n) {
+            // nothing fit: one unit alone exceeds the batch; grow it
+            (void)hipFree(r->batches.back());
+            r->batches.pop_back();
+            r->bases.pop_back();
+            base -= capacity;
+            capacity *= 2;
+            continue;
+        }
+        r->info.n_deferred += c.n_deferred;
+        if (c.n_deferred) {
+            // deferred[] indexes the current list; gather into the other one
+            uint2 *nxt = (cur == P.units) ? P.units_next : P.units;
+            HEGJ_TRY(gather_units(cur, P.deferred, c.n_deferred, nxt, s));
+            cur = nxt;
+        }
+        cur_n = c.n_deferred;
+    }
+    r->info.total_neighbors = c.pairs;
+    r->info.pair_hash = c.hash;
+    r->info.evals_executed = c.evals;
+    r->info.n_batches = (uint32_t)r->batches.size();
+    r->info.join_ms = join_ms;
+    r->info.join_launches = launches;
+    r->info.join_kernel_avg_ms = launches ? join_ms / launches : 0.0;
+    return HEGJ_OK;
+}
+
+// A FP64 join plan kept by the index together with a captured graph of one
+// batch: argument upload (from pinned host memory) -> counter reset (a fill
+// kernel) -> join kernel reading its arguments from device memory -> counter
+// download.  Replays rewrite the pinned argument block and launch the graph.
+}  // namespace
+
+struct hegj::GraphPlan {
+    uint32_t key[8] = {};
+    JoinPlan P;
+    uint32_t qb = 0, blocks = 0, grab = 1;
+    JoinArgs *d_args = nullptr, *h_args = nullptr;
+    Counters *h_ctr = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~GraphPlan() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (d_args) (void)hipFree(d_args);
+        if (h_args) (void)hipHostFree(h_args);
+        if (h_ctr) (void)hipHostFree(h_ctr);
+    }
+};
+
+void hegj::free_graph_plan(GraphPlan *g) { delete g; }
+
+namespace {
+
+hegj_status capture_graph(hegj_index *h, GraphPlan &G) {
+    hipStream_t s = h->stream;
+    JoinArgs proto{};
+    proto.g = h->g;  // selects the kernel instantiation
+    hipGraph_t graph = nullptr;
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hegj_status st = HEGJ_OK;
+    if (hipMemcpyAsync(G.d_args, G.h_args, sizeof(JoinArgs), hipMemcpyHostToDevice, s) != hipSuccess)
+        st = fail(HEGJ_ERR_HIP, "graph capture: argument upload");
+    if (st == HEGJ_OK) st = zero_words(G.P.d_ctr, sizeof(Counters), s);
+    if (st == HEGJ_OK) st = launch_join(proto, G.qb, G.blocks, s, G.d_args);
+    if (st == HEGJ_OK &&
+        hipMemcpyAsync(G.h_ctr, G.P.d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, s) != hipSuccess)
+        st = fail(HEGJ_ERR_HIP, "graph capture: counter download");
+    const hipError_t e = hipStreamEndCapture(s, &graph);
+    if (st != HEGJ_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    HIP_TRY(e);
+    const hipError_t ie = hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIP_TRY(ie);
+    return HEGJ_OK;
+}
+
+hegj_status selfjoin_graph(hegj_index *h, const hegj_join_opts &opts, uint32_t qb,
+                           const uint32_t *order, hegj_result *r, bool *fits) {
+    hipStream_t s = h->stream;
+    const uint32_t n = h->g.n;
+    const uint32_t key[8] = {qb, opts.unit_order, opts.xcd_parts, opts.units_per_grab,
+                             opts.part_by_block, (uint32_t)opts.result_capacity,
+                             (uint32_t)(opts.result_capacity >> 32), 1u};
+    if (h->gplan && std::memcmp(h->gplan->key, key, sizeof(key)) != 0) {
+        free_graph_plan(h->gplan);
+        h->gplan = nullptr;
+    }
+    if (!h->gplan) {
+        std::unique_ptr<GraphPlan> G(new GraphPlan());
+        std::memcpy(G->key, key, sizeof(key));
+        G->qb = qb;
+        HEGJ_TRY(make_plan(h, opts, qb, order, G->P));
+        G->grab = opts.units_per_grab ? opts.units_per_grab : grab_for(G->P, G->P.U);
+        G->blocks = blocks_for(G->P, G->P.U, G->grab);
+        HIP_TRY(hipMalloc(&G->d_args, sizeof(JoinArgs)));
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&G->h_args), sizeof(JoinArgs), hipHostMallocDefault));
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&G->h_ctr), sizeof(Counters), hipHostMallocDefault));
+        std::memset(G->h_args, 0, sizeof(JoinArgs));
+        HEGJ_TRY(capture_graph(h, *G));
+        h->gplan = G.release();
+    }
+    GraphPlan &G = *h->gplan;
+    r->info.n_units = G.P.U;
+    r->info.estimate_ms = G.P.estimate_ms;
+    HIP_TRY(hipMalloc(&r->d_row_off, sizeof(uint64_t) * n));
+    HIP_TRY(hipMalloc(&r->d_row_cnt, sizeof(uint32_t) * n));
+    HIP_TRY(hipMemsetAsync(r->d_row_off, 0, sizeof(uint64_t) * n, s));
+    HIP_TRY(hipMemsetAsync(r->d_row_cnt, 0, sizeof(uint32_t) * n, s));
+    uint32_t *buf = nullptr;
+    HIP_TRY(hipMalloc(&buf, sizeof(uint32_t) * G.P.capacity));
+    r->batches.push_back(buf);
+    r->bases.push_back(0);
+    JoinArgs a = base_args(h, opts, G.P);
+    a.row_off = r->d_row_off;
+    a.row_cnt = r->d_row_cnt;
+    a.units = G.P.units;
+    a.n_units = G.P.U;
+    a.units_per_grab = G.grab;
+    set_parts(a, G.P.parts, G.P.part_end);
+    a.out = buf;
+    a.out_base = 0;
+    a.capacity = G.P.capacity;
+    a.count_only = 0;
+    HIP_TRY(hipStreamSynchronize(s));  // the previous replay has read the block
+    *G.h_args = a;
+    {
+        EventTimer t(s);
+        HIP_TRY(hipGraphLaunch(G.exec, s));
+        r->info.join_ms = t.stop();  // synchronises the stream
+    }
+    const Counters c = *G.h_ctr;
+    if (c.n_deferred) {
+        // grow the cached capacity for later replays; this call runs eagerly
+        G.P.capacity = std::max<uint64_t>(G.P.capacity * 2, c.cursor + (1u << 20));
+        *fits = false;
+        return HEGJ_OK;
+    }
+    *fits = true;
+    r->info.total_neighbors = c.pairs;
+    r->info.pair_hash = c.hash;
+    r->info.evals_executed = c.evals;
+    r->info.n_batches = 1;
+    r->info.join_launches = 1;
+    r->info.join_kernel_avg_ms = r->info.join_ms;
+    r->info.graph_replay = 1;
+    return HEGJ_OK;
 }
 
 hegj_status ensure_gb(hegj_index *h) {
@@ -349,6 +672,7 @@ hegj_status hegj_index_free(hegj_index *h) {
     if (h->gb_A) (void)hipFree(h->gb_A);
     if (h->gb_B) (void)hipFree(h->gb_B);
     if (h->gb_w) (void)hipFree(h->gb_w);
+    if (h->gplan) free_graph_plan(h->gplan);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return HEGJ_OK;
@@ -444,6 +768,62 @@ void hegj_join_opts_default(hegj_join_opts *o) {
     o->unit_order = HEGJ_ORDER_MORTON;
 }
 
+static std::mutex g_tuning_mu;
+static std::map<uint32_t, hegj_join_opts> g_tuning;
+
+void hegj_join_opts_tuned(hegj_join_opts *o, uint32_t dim) {
+    hegj_join_opts_default(o);
+    std::lock_guard<std::mutex> lk(g_tuning_mu);
+    auto it = g_tuning.find(dim);
+    if (it != g_tuning.end()) *o = it->second;
+}
+
+hegj_status hegj_tuning_set(uint32_t dim, const hegj_join_opts *opts) {
+    if (!opts || dim == 0) return fail(HEGJ_ERR_INVALID_ARG, "NULL opts or dim 0");
+    if (opts->queries_per_unit != 0 && opts->queries_per_unit != 4 && opts->queries_per_unit != 8 &&
+        opts->queries_per_unit != 16)
+        return fail(HEGJ_ERR_INVALID_ARG, "queries_per_unit must be 0, 4, 8 or 16");
+    if (opts->unit_order > HEGJ_ORDER_MORTON || opts->refine > HEGJ_REFINE_MFMA)
+        return fail(HEGJ_ERR_INVALID_ARG, "unknown unit_order or refine");
+    std::lock_guard<std::mutex> lk(g_tuning_mu);
+    g_tuning[dim] = *opts;
+    return HEGJ_OK;
+}
+
+hegj_status hegj_tuning_load(const char *path, uint32_t *n_loaded) {
+    if (!path) return fail(HEGJ_ERR_INVALID_ARG, "NULL path");
+    FILE *f = std::fopen(path, "r");
+    if (!f) return fail(HEGJ_ERR_IO, std::string("cannot open ") + path);
+    char line[512];
+    uint32_t count = 0, lineno = 0;
+    hegj_status st = HEGJ_OK;
+    while (std::fgets(line, sizeof(line), f)) {
+        lineno++;
+        char *hash = std::strchr(line, '#');
+        if (hash) *hash = 0;
+        unsigned dim, qb, order, parts, grab, refine;
+        const int got = std::sscanf(line, "%u %u %u %u %u %u", &dim, &qb, &order, &parts, &grab, &refine);
+        if (got <= 0) continue;  // blank or comment
+        if (got != 6) {
+            st = fail(HEGJ_ERR_INVALID_ARG, "tuning table line " + std::to_string(lineno) + ": 6 fields expected");
+            break;
+        }
+        hegj_join_opts o;
+        hegj_join_opts_default(&o);
+        o.queries_per_unit = qb;
+        o.unit_order = order;
+        o.xcd_parts = parts;
+        o.units_per_grab = grab;
+        o.refine = refine;
+        st = hegj_tuning_set(dim, &o);
+        if (st != HEGJ_OK) break;
+        count++;
+    }
+    std::fclose(f);
+    if (n_loaded) *n_loaded = count;
+    return st;
+}
+
 hegj_status hegj_selfjoin(hegj_index *h, const hegj_join_opts *opts_in, hegj_result **out) {
     if (!h || !out) return fail(HEGJ_ERR_INVALID_ARG, "NULL argument");
     auto t_start = std::chrono::steady_clock::now();
@@ -452,7 +832,7 @@ hegj_status hegj_selfjoin(hegj_index *h, const hegj_join_opts *opts_in, hegj_res
     if (opts_in)
         opts = *opts_in;
     else
-        hegj_join_opts_default(&opts);
+        hegj_join_opts_tuned(&opts, h->g.dim);
     if (opts.refine > HEGJ_REFINE_MFMA) return fail(HEGJ_ERR_INVALID_ARG, "unknown refine");
     const bool mfma = opts.refine == HEGJ_REFINE_MFMA ||
                       (opts.refine == HEGJ_REFINE_AUTO && h->g.dim >= 16 && gb_kblocks(h->g.dim) != 0);
@@ -471,9 +851,6 @@ hegj_status hegj_selfjoin(hegj_index *h, const hegj_join_opts *opts_in, hegj_res
     r->info.workload_ms = h->workload_ms;
     r->info.n_queries = n - h->n_dead;
 
-    Scratch sc;
-    uint2 *units = nullptr;
-    uint32_t U = 0;
     const uint32_t *order = nullptr;
     if (opts.unit_order == HEGJ_ORDER_WORKLOAD) {
         order = h->d_order;
@@ -491,166 +868,25 @@ hegj_status hegj_selfjoin(hegj_index *h, const hegj_join_opts *opts_in, hegj_res
         return HEGJ_OK;
     }
     r->info.refine_used = HEGJ_REFINE_FP64;
-    HEGJ_TRY(build_units(h->g, h->ix, qb, order, &units, &U, s));
-    sc.ptrs.push_back(units);
-    r->info.n_units = U;
-
-    HIP_TRY(hipMalloc(&r->d_row_off, sizeof(uint64_t) * n));
-    HIP_TRY(hipMalloc(&r->d_row_cnt, sizeof(uint32_t) * n));
-    HIP_TRY(hipMemsetAsync(r->d_row_off, 0, sizeof(uint64_t) * n, s));
-    HIP_TRY(hipMemsetAsync(r->d_row_cnt, 0, sizeof(uint32_t) * n, s));
-
-    Counters *d_ctr;
-    uint32_t *deferred;
-    uint2 *units_next;
-    HIP_TRY(sc.get(&d_ctr, 1));
-    HIP_TRY(sc.get(&deferred, U + 1));
-    HIP_TRY(sc.get(&units_next, U + 1));
-
-    uint32_t max_blocks = 0;
-    HEGJ_TRY(join_occupancy(h->g.dim, qb, h->device, &max_blocks));
-    // per-wave hit-mask scratch for units whose masks exceed LDS
-    const uint32_t gmask_slots = 8192;
-    uint64_t *gmask;
-    HIP_TRY(sc.get(&gmask, (size_t)max_blocks * 4 * gmask_slots));
-
-    JoinArgs a{};
-    a.g = h->g;
-    a.ix = h->ix;
-    a.queue_heads = d_ctr->queue_heads;
-    a.n_deferred = &d_ctr->n_deferred;
-    a.cursor = &d_ctr->cursor;
-    a.hash = &d_ctr->hash;
-    a.evals = &d_ctr->evals;
-    a.pairs = &d_ctr->pairs;
-    a.row_off = r->d_row_off;
-    a.row_cnt = r->d_row_cnt;
-    a.deferred = deferred;
-    a.gmask = gmask;
-    a.gmask_slots = gmask_slots;
-    a.part_by_xcc = opts.part_by_block ? 0u : 1u;
-
-    auto set_parts = [&](JoinArgs &x, uint32_t nu, uint32_t parts, const uint32_t *ends) {
-        x.n_parts = parts;
-        for (uint32_t p = 0; p < parts; p++) {
-            x.part_begin[p] = p == 0 ? 0u : ends[p - 1];
-            x.part_end[p] = ends[p];
+    if (opts.use_graph) {
+        bool fits = false;
+        HEGJ_TRY(selfjoin_graph(h, opts, qb, order, r.get(), &fits));
+        if (!fits) {  // the cached capacity was too small: this call runs eagerly
+            r.reset(new hegj_result());
+            r->idx = h;
+            r->info.total_candidates = h->total_candidates;
+            r->info.workload_ms = h->workload_ms;
+            r->info.n_queries = n - h->n_dead;
+            r->info.refine_used = HEGJ_REFINE_FP64;
+            JoinPlan P;
+            HEGJ_TRY(make_plan(h, opts, qb, order, P));
+            HEGJ_TRY(run_batches(h, opts, qb, P, r.get()));
         }
-        (void)nu;
-    };
-    auto grab_for = [&](uint32_t nu) {
-        // about 32 grabs per wave keeps the queue atomics off the critical path
-        uint64_t waves = (uint64_t)max_blocks * 4;
-        uint64_t g = nu / (waves * 32);
-        return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, 64));
-    };
-    auto blocks_for = [&](uint32_t nu, uint32_t grab) {
-        uint64_t need = ((uint64_t)nu + grab * 4 - 1) / (grab * 4);
-        return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(need, max_blocks));
-    };
-    // unit ranges per XCD, split by estimated work
-    uint32_t parts = opts.xcd_parts ? std::min<uint32_t>(opts.xcd_parts, kMaxParts) : kMaxParts;
-    if (U < parts * 256) parts = 1;
-    uint32_t part_end[kMaxParts];
-    HEGJ_TRY(split_units(h->ix, units, U, qb, h->d_work, parts, part_end, s));
-
-    // -- batch estimate: count-only pass over a strided unit sample ------------
-    uint64_t capacity = opts.result_capacity;
-    if (capacity == 0 && U > 0) {
-        EventTimer t(s);
-        // ~8192 sampled units: a few per cent of the join's work at most
-        uint32_t stride = std::max<uint32_t>(1, U / 8192);
-        uint32_t ns = (U + stride - 1) / stride;
-        uint2 *sample;
-        HIP_TRY(sc.get(&sample, ns));
-        HEGJ_TRY(stride_units(units, ns, stride, sample, s));
-        HIP_TRY(hipMemsetAsync(d_ctr, 0, sizeof(Counters), s));
-        JoinArgs e = a;
-        e.units = sample;
-        e.n_units = ns;
-        e.units_per_grab = grab_for(ns);
-        e.count_only = 1;
-        e.gmask_slots = 0;
-        uint32_t one_end = ns;
-        set_parts(e, ns, 1, &one_end);
-        HEGJ_TRY(launch_join(e, qb, blocks_for(ns, e.units_per_grab), s));
-        Counters c;
-        HIP_TRY(hipMemcpyAsync(&c, d_ctr, sizeof(c), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        double est = (double)c.pairs * ((double)U / (double)ns);
-        capacity = (uint64_t)(est * 1.10) + (1u << 20);
-        size_t free_b = 0, total_b = 0;
-        HIP_TRY(hipMemGetInfo(&free_b, &total_b));
-        uint64_t cap_mem = (uint64_t)(free_b * 0.6) / sizeof(uint32_t);
-        capacity = std::min(capacity, cap_mem);
-        r->info.estimate_ms = t.stop();
+    } else {
+        JoinPlan P;
+        HEGJ_TRY(make_plan(h, opts, qb, order, P));
+        HEGJ_TRY(run_batches(h, opts, qb, P, r.get()));
     }
-    if (capacity == 0) capacity = 1;
-
-    // -- batches -----------------------------------------------------------------
-    HIP_TRY(hipMemsetAsync(d_ctr, 0, sizeof(Counters), s));
-    const uint2 *cur = units;
-    uint32_t cur_n = U;
-    uint64_t base = 0;
-    double join_ms = 0;
-    uint32_t launches = 0;
-    Counters c{};
-    while (cur_n > 0) {
-        uint32_t *buf = nullptr;
-        HIP_TRY(hipMalloc(&buf, sizeof(uint32_t) * capacity));
-        r->batches.push_back(buf);
-        r->bases.push_back(base);
-        // reset per-batch counters (hash / evals / pairs accumulate)
-        HIP_TRY(hipMemsetAsync(d_ctr, 0, offsetof(Counters, hash), s));
-        a.units = cur;
-        a.n_units = cur_n;
-        a.units_per_grab = opts.units_per_grab ? opts.units_per_grab : grab_for(cur_n);
-        if (cur == units && cur_n == U) {
-            set_parts(a, cur_n, parts, part_end);
-        } else {
-            uint32_t one_end = cur_n;
-            set_parts(a, cur_n, 1, &one_end);
-        }
-        a.out = buf;
-        a.out_base = base;
-        a.capacity = capacity;
-        a.count_only = 0;
-        {
-            EventTimer t(s);
-            HEGJ_TRY(launch_join(a, qb, blocks_for(cur_n, a.units_per_grab), s));
-            join_ms += t.stop();
-            launches++;
-        }
-        HIP_TRY(hipMemcpyAsync(&c, d_ctr, sizeof(c), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        if (r->batches.size() >= (size_t)kMaxBatches && c.n_deferred)
-            return fail(HEGJ_ERR_CAPACITY, "result needs more than 64 batches");
-        base += capacity;
-        if (c.n_deferred == cur_n) {
-            // nothing fit: one unit alone exceeds the batch; grow it
-            (void)hipFree(r->batches.back());
-            r->batches.pop_back();
-            r->bases.pop_back();
-            base -= capacity;
-            capacity *= 2;
-            continue;
-        }
-        r->info.n_deferred += c.n_deferred;
-        if (c.n_deferred) {
-            // deferred[] indexes the current list; gather into the other one
-            uint2 *nxt = (cur == units) ? units_next : units;
-            HEGJ_TRY(gather_units(cur, deferred, c.n_deferred, nxt, s));
-            cur = nxt;
-        }
-        cur_n = c.n_deferred;
-    }
-    r->info.total_neighbors = c.pairs;
-    r->info.pair_hash = c.hash;
-    r->info.evals_executed = c.evals;
-    r->info.n_batches = (uint32_t)r->batches.size();
-    r->info.join_ms = join_ms;
-    r->info.join_launches = launches;
-    r->info.join_kernel_avg_ms = launches ? join_ms / launches : 0.0;
     r->info.total_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     *out = r.release();

@@ -144,7 +144,12 @@ void free_index(DevIndex *ix);
 hegj_status launch_workload(const GridGeom &g, const DevIndex &ix, uint32_t *work, hipStream_t s);
 hegj_status build_units(const GridGeom &g, const DevIndex &ix, uint32_t qb, const uint32_t *order,
                         uint2 **units, uint32_t *n_units, hipStream_t s);
-hegj_status launch_join(const JoinArgs &a, uint32_t qb, uint32_t blocks, hipStream_t s);
+// dev_args != nullptr: the kernel reads its arguments from there (graph replay;
+// `a` then only selects the instantiation by a.g.dim).
+hegj_status launch_join(const JoinArgs &a, uint32_t qb, uint32_t blocks, hipStream_t s,
+                        const JoinArgs *dev_args = nullptr);
+// Zero `bytes` (a multiple of 4) with a kernel (safe inside graph capture).
+hegj_status zero_words(void *p, size_t bytes, hipStream_t s);
 // Workgroups of the persistent join kernel that fit on the device at once.
 hegj_status join_occupancy(uint32_t dim, uint32_t qb, int device, uint32_t *blocks);
 

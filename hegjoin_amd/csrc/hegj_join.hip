@@ -241,8 +241,20 @@ __device__ __forceinline__ bool next_grab(const JoinArgs &a, uint32_t &part, uin
     return false;
 }
 
+// IND: the arguments are read from device memory (ap) instead of the kernel
+// argument block, so one captured graph serves every call (the host rewrites
+// the arguments between replays).
+// Waves per SIMD the register budget must allow: 4 (<= 128 VGPRs) for the
+// small-D instances, whose loop is latency-bound and needs occupancy.
 template <int D, int QB>
-__global__ __launch_bounds__(64 * kWavesPerBlock) void k_join(JoinArgs a) {
+constexpr int join_min_waves() {
+    return (D > 0 && D <= 8 && QB <= 8) ? 4 : 1;
+}
+
+template <int D, int QB, bool IND = false>
+__global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB>())) void k_join(
+    JoinArgs av, const JoinArgs *__restrict__ ap) {
+    const JoinArgs &a = IND ? *ap : av;
     __shared__ RowList rls[kWavesPerBlock];
     __shared__ HitStore hs_all[kWavesPerBlock];
     extern __shared__ double q_dyn[];  // QB * dim query coordinates per wave
@@ -302,9 +314,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_join(JoinArgs a) {
             const uint32_t nmask = nq * nchunks;
             const bool keep = nmask <= a.gmask_slots || nmask <= (uint32_t)kMaskSlots;
             uint64_t *const hs = nmask <= (uint32_t)kMaskSlots ? hs_lds : hs_glb;
-            uint32_t cnt[QB];
-#pragma unroll
-            for (int qi = 0; qi < QB; qi++) cnt[qi] = 0;
+            uint32_t cnt_l = 0;  // lane qi: hits of query qi so far
 
             uint32_t slot = 0;
             // Candidate coordinates are prefetched kDepth chunks ahead in a
@@ -335,10 +345,15 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_join(JoinArgs a) {
                     if (kRegs) {
                         // use ring[p], then refill it with chunk ch + kDepth
                         // (no copy of registers that still have loads in flight)
-                        if constexpr (kQS)
+                        if constexpr (kQS) {
                             dist_regs<DR, QB>(ring[p], qreg, acc);
-                        else
-                            dist_regs_lds<DR, QB>(ring[p], qs, acc);
+                        } else {
+                            // opaque offset: keeps the compiler from hoisting
+                            // QB * D loop-invariant LDS reads into registers
+                            uint32_t qoff = (uint32_t)wid * QB * DR;
+                            __asm__ volatile("" : "+v"(qoff));
+                            dist_regs_lds<DR, QB>(ring[p], q_dyn + qoff, acc);
+                        }
                         {
                             const uint32_t tn = (ch + kDepth) * 64 + lane;
                             const bool vn = tn < M;
@@ -348,25 +363,30 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_join(JoinArgs a) {
                     } else {
                         const uint32_t t = ch * 64 + lane;
                         const uint32_t pos = valid ? cand_pos_adv(rl, nslots, t, slot) : 0u;
-                        dist_stream<D, QB>(a, pos, valid, qs, nq, acc);
+                        uint32_t qoff = (uint32_t)wid * QB * dimq;
+                        __asm__ volatile("" : "+v"(qoff));
+                        dist_stream<D, QB>(a, pos, valid, q_dyn + qoff, nq, acc);
                     }
                     uint64_t mine = 0;  // lane qi keeps query qi's mask
 #pragma unroll
                     for (int qi = 0; qi < QB; qi++) {
                         const uint64_t m = __ballot(valid && (uint32_t)qi < nq && acc[qi] <= T);
-                        cnt[qi] += (uint32_t)__popcll(m);
+                        cnt_l += lane == qi ? (uint32_t)__popcll(m) : 0u;
                         mine = lane == qi ? m : mine;
                     }
                     if (keep && (uint32_t)lane < nq && ch < nchunks) hs[lane * nchunks + ch] = mine;
                 }
             }
             my_evals += (uint64_t)M * nq;
-            uint32_t total = 0, excl[QB];
+            // lane qi: first slot of query qi within the unit's segment
+            uint32_t incl_l = cnt_l;
 #pragma unroll
-            for (int qi = 0; qi < QB; qi++) {
-                excl[qi] = total;
-                total += cnt[qi];
+            for (int o = 1; o < QB; o <<= 1) {
+                const uint32_t y = __shfl_up(incl_l, o, 64);
+                if (lane >= o) incl_l += y;
             }
+            const uint32_t excl_l = incl_l - cnt_l;
+            const uint32_t total = __shfl(incl_l, QB - 1, 64);
             if (a.count_only) {
                 my_pairs += total;
                 continue;
@@ -384,55 +404,58 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_join(JoinArgs a) {
             wave_lds_sync();
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             // row table entries of the unit's queries
+            const uint32_t qid_l = (uint32_t)lane < nq ? a.ix.ila[qpos0 + lane] : 0u;
             if ((uint32_t)lane < nq) {
-                uint32_t myexcl = 0, mycnt = 0;
-#pragma unroll
-                for (int qi = 0; qi < QB; qi++)
-                    if (qi == lane) {
-                        myexcl = excl[qi];
-                        mycnt = cnt[qi];
-                    }
-                const uint32_t qid = a.ix.ila[qpos0 + lane];
-                a.row_off[qid] = a.out_base + base + myexcl;
-                a.row_cnt[qid] = mycnt;
+                a.row_off[qid_l] = a.out_base + base + excl_l;
+                a.row_cnt[qid_l] = cnt_l;
             }
             if (total == 0) continue;
+            // Chunk-major write: one coalesced id load per chunk serves all of
+            // the unit's queries; lane qi keeps query qi's write cursor, and
+            // each query's neighbours come out in candidate (cell) order.
+            uint64_t o_l = base + excl_l;
+            uint32_t slot2 = 0;
+            for (uint32_t ch = 0; ch < nchunks; ch++) {
+                const uint32_t t = ch * 64 + lane;
+                const bool valid = t < M;
+                const uint32_t pos = valid ? cand_pos_adv(rl, nslots, t, slot2) : 0u;
+                const uint32_t id = valid ? a.ix.ila[pos] : 0u;
+                double x[DR];
+                if (!keep) {
+                    // masks did not fit anywhere (very large units): evaluate
+                    // the chunk again, one query at a time
+                    if constexpr (kRegs) load_cand<DR>(a, pos, valid, x);
+                    my_evals += (uint64_t)min(64u, M - ch * 64) * nq;
+                }
 #pragma unroll
-            for (int qi = 0; qi < QB; qi++) {
-                if ((uint32_t)qi >= nq || cnt[qi] == 0) continue;
-                const uint32_t qid = a.ix.ila[qpos0 + qi];
-                uint64_t o = base + excl[qi];
-                uint32_t slot2 = 0;
-                for (uint32_t ch = 0; ch < nchunks; ch++) {
-                    const uint32_t t = ch * 64 + lane;
-                    uint64_t m;
-                    if (keep) {
+                for (int qi = 0; qi < QB; qi++) {
+                    uint64_t m = 0;
+                    if ((uint32_t)qi >= nq) {
+                    } else if (keep) {
                         m = hs[qi * nchunks + ch];
                     } else {
-                        const bool valid = t < M;
-                        const uint32_t pos = valid ? cand_pos(rl, nslots, t) : 0u;
                         double acc1 = 0.0;
-                        if (kQS) {
-                            double x[DR];
-                            load_cand<DR>(a, pos, valid, x);
+                        if constexpr (kRegs) {
 #pragma unroll
                             for (int l = 0; l < DR; l++) {
-                                const double dd = x[l] - qreg[kQS ? qi : 0][l];
+                                const double q = kQS ? qreg[kQS ? qi : 0][kQS ? l : 0] : qs[qi * DR + l];
+                                const double dd = x[l] - q;
                                 acc1 = __builtin_fma(dd, dd, acc1);
                             }
                         } else {
                             dist_stream<D, 1>(a, pos, valid, qs + qi * dimq, 1, &acc1);
                         }
                         m = __ballot(valid && acc1 <= T);
-                        my_evals += (uint64_t)min(64u, M - ch * 64);
                     }
                     if (m == 0) continue;
+                    const uint64_t oq = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(o_l >> 32), qi) << 32) |
+                                        __builtin_amdgcn_readlane((uint32_t)o_l, qi);
+                    const uint32_t qid = __builtin_amdgcn_readlane(qid_l, qi);
                     if ((m >> lane) & 1ull) {
-                        const uint32_t id = a.ix.ila[cand_pos_adv(rl, nslots, t, slot2)];
-                        a.out[o + lanes_below(m)] = id;
+                        a.out[oq + lanes_below(m)] = id;
                         my_hash += mix64(((uint64_t)qid << 32) | id);
                     }
-                    o += (uint32_t)__popcll(m);
+                    o_l += lane == qi ? (uint64_t)__popcll(m) : 0ull;
                 }
             }
         }
@@ -529,33 +552,50 @@ hegj_status build_units(const GridGeom &g, const DevIndex &ix, uint32_t qb, cons
 }
 
 template <int D, int QB>
-static hegj_status launch_join_t(const JoinArgs &a, uint32_t blocks, hipStream_t s) {
+static hegj_status launch_join_t(const JoinArgs &a, const JoinArgs *ap, uint32_t blocks,
+                                 hipStream_t s) {
     const uint32_t lds = q_lds_bytes<D, QB>(a.g.dim);
     if (lds > kMaxQueryLds)
         return fail(HEGJ_ERR_UNSUPPORTED, "queries_per_unit * dim too large for LDS; lower queries_per_unit");
-    k_join<D, QB><<<blocks, 64 * kWavesPerBlock, lds, s>>>(a);
+    if (ap)
+        k_join<D, QB, true><<<blocks, 64 * kWavesPerBlock, lds, s>>>(JoinArgs{}, ap);
+    else
+        k_join<D, QB, false><<<blocks, 64 * kWavesPerBlock, lds, s>>>(a, nullptr);
     HIP_TRY(hipGetLastError());
     return HEGJ_OK;
 }
 
 template <int QB>
-static hegj_status launch_join_qb(const JoinArgs &a, uint32_t blocks, hipStream_t s) {
+static hegj_status launch_join_qb(const JoinArgs &a, const JoinArgs *ap, uint32_t blocks,
+                                  hipStream_t s) {
     switch (a.g.dim) {
-        case 2: return launch_join_t<2, QB>(a, blocks, s);
-        case 6: return launch_join_t<6, QB>(a, blocks, s);
-        case 18: return launch_join_t<18, QB>(a, blocks, s);
-        case 90: return launch_join_t<90, QB>(a, blocks, s);
-        default: return launch_join_t<0, QB>(a, blocks, s);
+        case 2: return launch_join_t<2, QB>(a, ap, blocks, s);
+        case 6: return launch_join_t<6, QB>(a, ap, blocks, s);
+        case 18: return launch_join_t<18, QB>(a, ap, blocks, s);
+        case 90: return launch_join_t<90, QB>(a, ap, blocks, s);
+        default: return launch_join_t<0, QB>(a, ap, blocks, s);
     }
 }
 
-hegj_status launch_join(const JoinArgs &a, uint32_t qb, uint32_t blocks, hipStream_t s) {
+hegj_status launch_join(const JoinArgs &a, uint32_t qb, uint32_t blocks, hipStream_t s,
+                        const JoinArgs *dev_args) {
     switch (qb) {
-        case 4: return launch_join_qb<4>(a, blocks, s);
-        case 8: return launch_join_qb<8>(a, blocks, s);
-        case 16: return launch_join_qb<16>(a, blocks, s);
+        case 4: return launch_join_qb<4>(a, dev_args, blocks, s);
+        case 8: return launch_join_qb<8>(a, dev_args, blocks, s);
+        case 16: return launch_join_qb<16>(a, dev_args, blocks, s);
         default: return fail(HEGJ_ERR_INVALID_ARG, "queries_per_unit must be 4, 8 or 16");
     }
+}
+
+__global__ void k_zero_words(uint32_t *__restrict__ p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+hegj_status zero_words(void *p, size_t bytes, hipStream_t s) {
+    const uint32_t n = (uint32_t)(bytes / 4);
+    k_zero_words<<<std::min<uint32_t>(ceil_div(n, 256), 1024), 256, 0, s>>>(static_cast<uint32_t *>(p), n);
+    HIP_TRY(hipGetLastError());
+    return HEGJ_OK;
 }
 
 }  // namespace hegj
@@ -566,7 +606,7 @@ hegj_status join_occupancy(uint32_t dim, uint32_t qb, int device, uint32_t *bloc
     int per_cu = 0;
     const void *fn = nullptr;
     uint32_t lds = 0;
-#define HEGJ_OCC(D, Q) fn = reinterpret_cast<const void *>(&k_join<D, Q>), lds = q_lds_bytes<D, Q>(dim)
+#define HEGJ_OCC(D, Q) fn = reinterpret_cast<const void *>(&k_join<D, Q, false>), lds = q_lds_bytes<D, Q>(dim)
     uint32_t d = (dim == 2 || dim == 6 || dim == 18 || dim == 90) ? dim : 0;
     switch (qb * 1000 + d) {
         case 4002: HEGJ_OCC(2, 4); break;

@@ -16,6 +16,9 @@
 //   --dump-csr=PATH     write the neighbour table (rows sorted) as binary:
 //                       uint32 n, uint64 offsets[n+1], uint32 neighbours[]
 //   --device=D          HIP device ordinal
+//   --tuning=PATH       join options per dimensionality (autotuner table);
+//                       by default tuning.tsv next to this executable is used
+//                       when present
 #include <cerrno>
 #include <chrono>
 #include <cstdint>
@@ -45,7 +48,7 @@ int main(int argc, char **argv) {
     std::vector<char *> pos;
     uint32_t kOverride = 0;
     int emulate = 1, device = 0;
-    const char *dump = nullptr;
+    const char *dump = nullptr, *tuning = nullptr;
     for (int i = 1; i < argc; i++) {
         if (std::strncmp(argv[i], "--indexed-dims=", 15) == 0)
             kOverride = (uint32_t)std::atoi(argv[i] + 15);
@@ -55,8 +58,28 @@ int main(int argc, char **argv) {
             dump = argv[i] + 11;
         else if (std::strncmp(argv[i], "--device=", 9) == 0)
             device = std::atoi(argv[i] + 9);
+        else if (std::strncmp(argv[i], "--tuning=", 9) == 0)
+            tuning = argv[i] + 9;
         else
             pos.push_back(argv[i]);
+    }
+    {
+        std::string path = tuning ? tuning : "";
+        if (path.empty()) {  // tuning.tsv beside the executable
+            std::string self = argv[0];
+            const size_t slash = self.rfind('/');
+            path = (slash == std::string::npos ? std::string(".") : self.substr(0, slash)) + "/tuning.tsv";
+            FILE *f = std::fopen(path.c_str(), "r");
+            if (f)
+                std::fclose(f);
+            else
+                path.clear();
+        }
+        uint32_t loaded = 0;
+        if (!path.empty() && hegj_tuning_load(path.c_str(), &loaded) != HEGJ_OK) {
+            std::fprintf(stderr, "[MAIN] ~ Error: tuning table %s: %s\n", path.c_str(), hegj_last_error());
+            return 1;
+        }
     }
     int nargs = (int)pos.size() + 1;
     if (NB_ARGS_MAX != nargs && (NB_ARGS_MAX - 1) != nargs) {

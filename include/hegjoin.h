@@ -178,9 +178,21 @@ typedef struct {
     uint32_t xcd_parts;       /* unit ranges, one per XCD: 0 = auto (8), 1 = one queue */
     uint32_t units_per_grab;  /* units a wave takes per queue atomic; 0 = auto */
     uint32_t part_by_block;   /* 0: range by hardware XCD id, 1: by workgroup index */
+    uint32_t use_graph;       /* FP64 refine: keep the plan in the index and run the
+                                 batch as a replay of a captured hipGraph */
 } hegj_join_opts;
 
 void hegj_join_opts_default(hegj_join_opts *opts);
+/* Defaults for a dimensionality: hegj_join_opts_default, then the tuned
+ * entry for `dim` (hegj_tuning_set / hegj_tuning_load) when there is one.
+ * hegj_selfjoin with opts == NULL uses these. */
+void hegj_join_opts_tuned(hegj_join_opts *opts, uint32_t dim);
+/* Register tuned join options for one dimensionality (replaces any entry). */
+hegj_status hegj_tuning_set(uint32_t dim, const hegj_join_opts *opts);
+/* Load a tuning table written by hegjoin_amd/autotune.py: one line per
+ * dimensionality, "dim queries_per_unit unit_order xcd_parts units_per_grab
+ * refine", '#' starts a comment.  *n_loaded (may be NULL) = entries read. */
+hegj_status hegj_tuning_load(const char *path, uint32_t *n_loaded);
 
 hegj_status hegj_selfjoin(hegj_index *idx, const hegj_join_opts *opts, hegj_result **out);
 hegj_status hegj_result_free(hegj_result *res);
@@ -202,6 +214,8 @@ typedef struct {
     double join_kernel_avg_ms; /* average duration of one join kernel launch */
     uint32_t join_launches;
     uint32_t refine_used;      /* hegj_refine that ran: FP64 or MFMA */
+    uint32_t graph_replay;     /* 1: the join ran as a replay of the captured graph */
+    uint32_t reserved;
 } hegj_result_info;
 hegj_status hegj_result_get_info(const hegj_result *res, hegj_result_info *info);
 

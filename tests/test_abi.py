@@ -53,8 +53,8 @@ PROBE = r"""
 int main(void) {
     S(hegj_index_opts) S(hegj_index_info) S(hegj_join_opts) S(hegj_result_info)
     S(hegj_serve_opts) S(hegj_request_info) S(hegj_serve_stats)
-    O(hegj_index_info, build_ms) O(hegj_join_opts, part_by_block)
-    O(hegj_result_info, refine_used) O(hegj_serve_opts, result_bytes)
+    O(hegj_index_info, build_ms) O(hegj_join_opts, part_by_block) O(hegj_join_opts, use_graph)
+    O(hegj_result_info, refine_used) O(hegj_result_info, graph_replay) O(hegj_serve_opts, result_bytes)
     O(hegj_serve_opts, device) O(hegj_request_info, latency_ms) O(hegj_serve_stats, wall_ms)
     return 0;
 }
@@ -196,3 +196,46 @@ def test_dataset_loader_matches_oracle(oracle, tmp_path):
     p = str(tmp_path / "r.txt")
     open(p, "w").write("0.1, 0.2\n0.3,0.4\n 0.5 ,\t0.6\n0.7\n")
     assert H.load_dataset(p, 2).tolist() == [[0.1, 0.2], [0.3, 0.4], [0.5, 0.6]]
+
+
+def test_tuning_table_roundtrip(tmp_path):
+    """autotune.write_tables -> hegj_tuning_load -> hegj_join_opts_tuned."""
+    import hegjoin_amd as H
+    from hegjoin_amd import autotune as T
+    best = dict(refine="fp64", unit_order="workload", queries_per_unit=8, xcd_parts=1,
+                units_per_grab=4, ms=1.0)
+    tsv, js = T.write_tables(7, dict(dim=7), best, [best], str(tmp_path / "t.tsv"),
+                             str(tmp_path / "t.json"))
+    assert H.load_tuning(tsv) == 1
+    o = H._JoinOpts()
+    H.library().hegj_join_opts_tuned(C.byref(o), 7)
+    assert (o.queries_per_unit, o.unit_order, o.xcd_parts, o.units_per_grab, o.refine) == (8, 1, 1, 4, 1)
+    H.library().hegj_join_opts_tuned(C.byref(o), 5)  # no entry: plain defaults
+    assert (o.queries_per_unit, o.unit_order) == (0, 2)
+    bad = tmp_path / "bad.tsv"
+    bad.write_text("# comment\n3 4 2 1\n")
+    with pytest.raises(H.HegjError):
+        H.load_tuning(str(bad))
+    bad.write_text("3 5 2 1 0 1\n")  # queries_per_unit 5 is not a kernel
+    with pytest.raises(H.HegjError):
+        H.load_tuning(str(bad))
+    with pytest.raises(H.HegjError):
+        H.load_tuning(str(tmp_path / "missing.tsv"))
+
+
+def test_bench_rooflines_pick_the_binding_roof():
+    """2-D (few flops per pair, many pairs) binds on HBM; 6-D FP64 on compute;
+    the MFMA refine uses 2D flops per pair against bf16 peak / 3."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(ROOT, "bench.py"))
+    B = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(B)
+    roof, other = B.rooflines(1_000_000, 2, 20_000, 1.06e9, 3.0e9, 10.0, "fp64")
+    assert roof["bound"] == "hbm" and other["bound"] == "mfma"
+    assert abs(roof["work"] - (8 * 2e6 + 4e6 + 24 * 2e4 + 8 * 1.06e9)) < 1
+    roof, other = B.rooflines(5_000_000, 6, 1_860_000, 3.0e8, 2.4e10, 100.0, "fp64")
+    assert roof["bound"] == "mfma" and abs(roof["peak"] - 78.6) < 1e-9
+    assert abs(roof["achieved"] - 3 * 6 * 2.4e10 / 0.1 / 1e12) < 1e-9
+    roof, other = B.rooflines(2_000_000, 18, 2900, 7.4e7, 2.8e12, 500.0, "mfma")
+    assert roof["bound"] == "mfma" and abs(roof["peak"] - 2500 / 3) < 1e-9
+    assert abs(roof["frac"] - 2 * 18 * 2.8e12 / 0.5 / 1e12 / (2500 / 3)) < 1e-12

@@ -97,6 +97,49 @@ def test_join_variants_identical(H, qb, order, parts):
     assert info["pair_hash"] == int(g["hash"])
 
 
+def test_graph_replay_matches_eager(H, oracle):
+    """The captured batch graph, replayed several times with eager joins and
+    synchronisations in between, gives the eager result every time."""
+    pts = oracle.generate_expo(30000, 6, seed=77)
+    eoff, enb = oracle.join(pts, 6, 0.012)
+    with H.GridIndex(pts, 0.012, indexed_dims=6) as ix:
+        for i in range(4):
+            t = ix.selfjoin(use_graph=True)
+            info = t.info
+            off, nb = t.csr(sort_rows=True)
+            t.close()
+            assert info["graph_replay"] == 1
+            assert np.array_equal(off, eoff) and np.array_equal(nb, enb), i
+            assert info["pair_hash"] == oracle.csr_hash(eoff, enb)
+            e = ix.selfjoin(queries_per_unit=8)  # eager work between replays
+            assert e.info["pair_hash"] == info["pair_hash"]
+            e.close()
+        # other options: the plan is rebuilt and captured again
+        t = ix.selfjoin(use_graph=True, queries_per_unit=8, unit_order="workload")
+        off, nb = t.csr(sort_rows=True)
+        assert t.info["graph_replay"] == 1
+        t.close()
+        assert np.array_equal(off, eoff) and np.array_equal(nb, enb)
+
+
+def test_graph_replay_with_small_capacity_falls_back(H, oracle):
+    """A fixed capacity too small for one batch: the replay defers, the call
+    runs eagerly in batches; the plan's capacity grows, so the next call
+    replays the graph.  The result never changes."""
+    g = load_golden(golden_files("alg0_d6_")[0])
+    with H.GridIndex(g["pts"], float(g["eps"]), indexed_dims=int(g["k"])) as ix:
+        for i in range(3):
+            t = ix.selfjoin(use_graph=True, result_capacity=5000)
+            off, nb = t.csr(sort_rows=True)
+            info = t.info
+            t.close()
+            if i == 0:
+                assert info["graph_replay"] == 0 and info["n_batches"] > 1
+            else:
+                assert info["graph_replay"] == 1 and info["n_batches"] == 1
+            assert np.array_equal(off, g["offsets"]) and np.array_equal(nb, g["nbrs"])
+
+
 @pytest.mark.parametrize("cap", [5000, 40000])
 def test_small_batches_defer_and_match(H, cap):
     """A result batch far smaller than the output forces many batches and
