@@ -124,14 +124,31 @@ def rooflines(n, dim, cells, pairs, evals, kernel_ms, refine):
     return (comp, hbm) if comp["roof_ms"] >= hbm["roof_ms"] else (hbm, comp)
 
 
-def serve_levels(eps_lo, eps_hi):
+def attach_traffic(roof, config, refine, kernel_ms):
+    """HBM traffic per launch of the dominant kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same workload (each
+    counter scaled by its reading of the bench's 1 GiB copy kernel)."""
+    tag = config + ("_mfma" if refine == "mfma" else "")
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return
+    with open(path) as f:
+        t = json.load(f).get(tag)
+    if t:
+        roof["traffic"] = t["traffic_bytes"]
+        roof["traffic_source"] = (f"{t['profile']} (profiled launch {t['kernel_avg_ms']:.2f} ms; "
+                                  f"this run {kernel_ms:.2f} ms)")
+
+
+def serve_levels(eps_lo, eps_hi, ratio=2 ** 0.25, max_levels=16):
     """Grid levels for the serving index: cell sides from just above eps_hi
-    down by factors of sqrt(2) while they still exceed eps_lo / sqrt(2)."""
+    down by `ratio` while they still exceed eps_lo / ratio, so a request's
+    cell side is at most `ratio` times its eps."""
     top = eps_hi * (1 + 1e-6)
     sides = [top]
-    while sides[-1] / np.sqrt(2) >= eps_lo * 0.999:
-        sides.append(sides[-1] / np.sqrt(2))
-    return sorted(sides)[:8]
+    while sides[-1] / ratio >= eps_lo * 0.999 and len(sides) < max_levels:
+        sides.append(sides[-1] / ratio)
+    return sorted(sides)
 
 
 def serve_main(a):
@@ -170,21 +187,31 @@ def serve_main(a):
                 ready[j].set()
         th = threading.Thread(target=submitter)
         th.start()
-        infos = []
-        for j in range(hi - lo):
-            ready[j].wait()
-            try:
-                info = srv.wait(tickets[j])
-                info["ok"] = True
-            except H.HegjError as e:
-                info = {"ok": False, "error": str(e), "n_queries": int(nqs[lo + j])}
-            info["eps"] = float(epss[lo + j])
-            srv.release(tickets[j])
-            infos.append(info)
-            if timed:
-                print(f"# request {j}: nq {info['n_queries']} eps {info['eps']:.4f} "
-                      + (f"pairs {info['pairs']} latency {info['latency_ms']:.1f} ms" if info["ok"]
-                         else "rejected: " + info["error"]), file=sys.stderr, flush=True)
+        # collect in completion order and release at once (a client that held
+        # finished results while waiting for another request could starve
+        # admission of the pool)
+        infos, left = [], list(range(hi - lo))
+        while left:
+            for j in list(left):
+                if not ready[j].is_set():
+                    continue
+                try:
+                    info = srv.poll(tickets[j])
+                    if info is None:
+                        continue
+                    info["ok"] = True
+                except H.HegjError as e:
+                    info = {"ok": False, "error": str(e), "n_queries": int(nqs[lo + j])}
+                info["eps"] = float(epss[lo + j])
+                srv.release(tickets[j])
+                infos.append(info)
+                left.remove(j)
+                if timed:
+                    print(f"# request {j}: nq {info['n_queries']} eps {info['eps']:.4f} "
+                          + (f"pairs {info['pairs']} latency {info['latency_ms']:.1f} ms"
+                             if info["ok"] else "rejected: " + info["error"]),
+                          file=sys.stderr, flush=True)
+            time.sleep(0.0005)
         th.join()
         return infos, time.perf_counter() - start
 
@@ -297,6 +324,7 @@ def main():
     roof, roof_alt = rooflines(n, dim, iinfo["nNonEmptyCells"], pairs, total_cand / per_launch,
                                kernel_ms, refine_used)
     roof_alt["attainable_copy_gbs"] = copy_gbs
+    attach_traffic(roof, a.config, refine_used, kernel_ms)
     out = {
         "metric": "result pairs/s (1 GPU)",
         "value": pairs * a.steps / elapsed,

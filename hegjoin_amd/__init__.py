@@ -79,7 +79,7 @@ class _ResultInfo(C.Structure):
 
 class _ServeOpts(C.Structure):
     _fields_ = [("dim", C.c_uint32), ("indexed_dims", C.c_uint32), ("n_levels", C.c_uint32),
-                ("slots", C.c_uint32), ("cell_sides", C.c_double * 8),
+                ("slots", C.c_uint32), ("cell_sides", C.c_double * 16),
                 ("result_bytes", C.c_uint64), ("max_queries", C.c_uint32), ("device", C.c_int32)]
 
 
@@ -141,6 +141,7 @@ _lib.hegj_tuning_load.argtypes = [C.c_char_p, C.POINTER(C.c_uint32)]
 _lib.hegj_serve_open.argtypes = [_vp, C.c_uint32, C.POINTER(_ServeOpts), C.POINTER(_vp)]
 _lib.hegj_serve_submit.argtypes = [_vp, _vp, C.c_uint32, C.c_double, C.POINTER(C.c_uint64)]
 _lib.hegj_serve_wait.argtypes = [_vp, C.c_uint64, C.POINTER(_RequestInfo)]
+_lib.hegj_serve_poll.argtypes = [_vp, C.c_uint64, C.POINTER(_RequestInfo)]
 _lib.hegj_serve_copy.argtypes = [_vp, C.c_uint64, _vp, _vp]
 _lib.hegj_serve_release.argtypes = [_vp, C.c_uint64]
 _lib.hegj_serve_get_stats.argtypes = [_vp, C.POINTER(_ServeStats), C.c_int]
@@ -381,8 +382,8 @@ class Server:
         o.dim = dim
         o.indexed_dims = indexed_dims if indexed_dims is not None else min(dim, 6)
         sides = list(cell_sides)
-        if not 1 <= len(sides) <= 8:
-            raise ValueError("1 to 8 cell sides")
+        if not 1 <= len(sides) <= 16:
+            raise ValueError("1 to 16 cell sides")
         o.n_levels = len(sides)
         for i, c in enumerate(sides):
             o.cell_sides[i] = float(c)
@@ -408,6 +409,16 @@ class Server:
         HegjError when the request failed)."""
         info = _RequestInfo()
         _check(_lib.hegj_serve_wait(self._h, ticket, C.byref(info)))
+        return _struct_dict(info)
+
+    def poll(self, ticket):
+        """The request's info dict when it has finished, None while it runs
+        (raises HegjError when it failed)."""
+        info = _RequestInfo()
+        st = _lib.hegj_serve_poll(self._h, ticket, C.byref(info))
+        if st == 8:  # HEGJ_ERR_BUSY
+            return None
+        _check(st)
         return _struct_dict(info)
 
     def result(self, ticket, release=True):

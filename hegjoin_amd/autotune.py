@@ -31,16 +31,19 @@ REFINES = {"auto": 0, "fp64": 1, "mfma": 2}
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def candidates(dim):
-    """Option sets to try for one dimensionality."""
+def candidates(dim, with_fp64=False):
+    """Option sets to try for one dimensionality.  At 16 dims and above the
+    MFMA refine is swept; the FP64 kernel only with with_fp64 (at the full
+    18-D/90-D sizes one FP64 join takes minutes)."""
     out = []
     if dim >= 16:
         for order in ("morton", "workload", "spatial"):
             out.append(dict(refine="mfma", unit_order=order, queries_per_unit=0, xcd_parts=1,
                             units_per_grab=0))
-        for qb in (4, 8):
-            out.append(dict(refine="fp64", unit_order="morton", queries_per_unit=qb, xcd_parts=8,
-                            units_per_grab=0))
+        if with_fp64:
+            for qb in (4, 8):
+                out.append(dict(refine="fp64", unit_order="morton", queries_per_unit=qb,
+                                xcd_parts=8, units_per_grab=0))
         return out
     for qb, order, parts, grab in itertools.product((4, 8, 16), ("morton", "workload", "spatial"),
                                                     (1, 8), (0, 1, 4)):
@@ -66,14 +69,14 @@ def measure(ix, opts, steps, limit_s):
     return sum(times) / len(times), info, len(times)
 
 
-def tune(dim, n, eps, k=None, steps=2, limit_s=20.0, log=print):
+def tune(dim, n, eps, k=None, steps=2, limit_s=20.0, log=print, with_fp64=False):
     import hegjoin_amd as H
     pts = H.generate_expo(n, dim, seed=0)
     k = k or min(dim, 6)
     results = []
     ref = None
     with H.GridIndex(pts, eps, indexed_dims=k) as ix:
-        for opts in candidates(dim):
+        for opts in candidates(dim, with_fp64):
             try:
                 ms, info, runs = measure(ix, opts, steps, limit_s)
             except H.HegjError as e:
@@ -121,12 +124,13 @@ def main(argv=None):
     ap.add_argument("--k", type=int, default=None)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--limit-s", type=float, default=20.0)
+    ap.add_argument("--with-fp64", action="store_true", help="also time the FP64 kernel at dim >= 16")
     ap.add_argument("--tsv", default=None)
     ap.add_argument("--json", default=None)
     a = ap.parse_args(argv)
     t0 = time.time()
     best, results = tune(a.dim, a.n, a.eps, a.k, a.steps, a.limit_s,
-                         log=lambda s: print(s, file=sys.stderr, flush=True))
+                         log=lambda s: print(s, file=sys.stderr, flush=True), with_fp64=a.with_fp64)
     workload = dict(dim=a.dim, n=a.n, eps=a.eps, k=a.k or min(a.dim, 6), generator="expo seed 0")
     tsv, js = write_tables(a.dim, workload, best, results, a.tsv, a.json)
     print(json.dumps(dict(best=best, configs=len(results), seconds=time.time() - t0, tsv=tsv)))

@@ -225,6 +225,7 @@ struct JoinPlan {
     uint32_t U = 0;
     uint32_t *deferred = nullptr;
     Counters *d_ctr = nullptr;
+    Counters *d_ctr2 = nullptr;  // count-only passes over deferred units
     uint64_t *gmask = nullptr;
     uint32_t gmask_slots = 8192;
     uint32_t max_blocks = 0;
@@ -236,7 +237,8 @@ struct JoinPlan {
     JoinPlan(const JoinPlan &) = delete;
     JoinPlan &operator=(const JoinPlan &) = delete;
     ~JoinPlan() {
-        for (void *p : {(void *)units, (void *)units_next, (void *)deferred, (void *)d_ctr, (void *)gmask})
+        for (void *p : {(void *)units, (void *)units_next, (void *)deferred, (void *)d_ctr,
+                        (void *)d_ctr2, (void *)gmask})
             if (p) (void)hipFree(p);
     }
 };
@@ -286,6 +288,7 @@ hegj_status make_plan(hegj_index *h, const hegj_join_opts &opts, uint32_t qb, co
     HIP_TRY(hipMalloc(&P.units_next, sizeof(uint2) * (U + 1)));
     HIP_TRY(hipMalloc(&P.deferred, sizeof(uint32_t) * (U + 1)));
     HIP_TRY(hipMalloc(&P.d_ctr, sizeof(Counters)));
+    HIP_TRY(hipMalloc(&P.d_ctr2, sizeof(Counters)));
     HEGJ_TRY(join_occupancy(h->g.dim, qb, h->device, &P.max_blocks));
     // per-wave hit-mask scratch for units whose masks exceed LDS
     HIP_TRY(hipMalloc(&P.gmask, sizeof(uint64_t) * (size_t)P.max_blocks * 4 * P.gmask_slots));
@@ -396,6 +399,27 @@ hegj_status run_batches(hegj_index *h, const hegj_join_opts &opts, uint32_t qb, 
             uint2 *nxt = (cur == P.units) ? P.units_next : P.units;
             HEGJ_TRY(gather_units(cur, P.deferred, c.n_deferred, nxt, s));
             cur = nxt;
+            // size the next batch exactly: count-only pass over what is left
+            JoinArgs e = a;
+            e.units = cur;
+            e.n_units = c.n_deferred;
+            e.units_per_grab = grab_for(P, e.n_units);
+            uint32_t one_end = e.n_units;
+            set_parts(e, 1, &one_end);
+            e.count_only = 1;
+            e.gmask_slots = 0;
+            e.queue_heads = P.d_ctr2->queue_heads;
+            e.n_deferred = &P.d_ctr2->n_deferred;
+            e.cursor = &P.d_ctr2->cursor;
+            e.hash = &P.d_ctr2->hash;
+            e.evals = &P.d_ctr2->evals;
+            e.pairs = &P.d_ctr2->pairs;
+            HIP_TRY(hipMemsetAsync(P.d_ctr2, 0, sizeof(Counters), s));
+            HEGJ_TRY(launch_join(e, qb, blocks_for(P, e.n_units, e.units_per_grab), s));
+            Counters c2;
+            HIP_TRY(hipMemcpyAsync(&c2, P.d_ctr2, sizeof(c2), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            if (!opts.result_capacity) capacity = std::max<uint64_t>(c2.pairs, 1);
         }
         cur_n = c.n_deferred;
     }

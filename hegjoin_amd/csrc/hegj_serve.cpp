@@ -21,7 +21,6 @@
 #include <map>
 #include <memory>
 #include <mutex>
-#include <set>
 #include <thread>
 #include <vector>
 
@@ -115,10 +114,9 @@ struct hegj_server {
     std::mutex mu;
     std::condition_variable cv_work, cv_done, cv_mem;
     std::deque<Request *> pending;
-    // tickets (with queries) not yet through admission: admission is in
-    // ticket order, so a client that releases results in ticket order can
-    // never wait on memory held by a later request
-    std::set<uint64_t> unadmitted;
+    // counted requests waiting for pool memory, served first come, first
+    // served (a request still counting does not hold up later ones)
+    std::deque<Request *> mem_wait;
     std::map<uint64_t, std::unique_ptr<Request>> reqs;
     uint64_t next_ticket = 1;
     bool stopping = false;
@@ -143,10 +141,9 @@ hegj_status admit(hegj_server *S, Request *r, uint64_t bytes) {
         return fail(HEGJ_ERR_CAPACITY, "request result (" + std::to_string(bytes) +
                                            " bytes) exceeds the result-buffer pool");
     uint64_t off = 0;
-    S->cv_mem.wait(lk, [&] {
-        return *S->unadmitted.begin() == r->ticket && S->pool.alloc(bytes, &off);
-    });
-    S->unadmitted.erase(r->ticket);
+    S->mem_wait.push_back(r);
+    S->cv_mem.wait(lk, [&] { return S->mem_wait.front() == r && S->pool.alloc(bytes, &off); });
+    S->mem_wait.pop_front();
     r->holds_pool = true;
     r->pool_off = off;
     r->pool_bytes = bytes;
@@ -257,8 +254,6 @@ void hegj_server::worker(uint32_t slot) {
                 r->status = st;
                 r->error = msg;
                 failed++;
-                unadmitted.erase(r->ticket);
-                cv_mem.notify_all();
                 if (r->holds_pool) {
                     pool.release(r->pool_off, r->pool_bytes);
                     r->holds_pool = false;
@@ -301,7 +296,7 @@ hegj_status hegj_serve_open(const double *database, uint32_t n, const hegj_serve
     if (o->indexed_dims == 0 || o->indexed_dims > HEGJ_MAX_INDEXED_DIMS || o->indexed_dims > o->dim)
         return fail(HEGJ_ERR_INVALID_ARG, "indexed_dims must be in [1, min(dim, 6)]");
     if (o->n_levels == 0 || o->n_levels > HEGJ_SERVE_MAX_LEVELS)
-        return fail(HEGJ_ERR_INVALID_ARG, "n_levels must be in [1, 8]");
+        return fail(HEGJ_ERR_INVALID_ARG, "n_levels must be in [1, 16]");
     if (o->slots == 0 || o->slots > 16) return fail(HEGJ_ERR_INVALID_ARG, "slots must be in [1, 16]");
     if (o->max_queries == 0) return fail(HEGJ_ERR_INVALID_ARG, "max_queries must be >= 1");
     if (o->result_bytes < 4096) return fail(HEGJ_ERR_INVALID_ARG, "result_bytes must be >= 4096");
@@ -398,7 +393,6 @@ hegj_status hegj_serve_submit(hegj_server *S, const double *queries, uint32_t nq
             S->lat.push_back(0.0);
         } else {
             S->pending.push_back(raw);
-            S->unadmitted.insert(raw->ticket);
         }
     }
     S->cv_work.notify_one();
@@ -406,13 +400,7 @@ hegj_status hegj_serve_submit(hegj_server *S, const double *queries, uint32_t nq
     return HEGJ_OK;
 }
 
-hegj_status hegj_serve_wait(hegj_server *S, uint64_t ticket, hegj_request_info *info) {
-    if (!S) return fail(HEGJ_ERR_INVALID_ARG, "NULL server");
-    std::unique_lock<std::mutex> lk(S->mu);
-    auto it = S->reqs.find(ticket);
-    if (it == S->reqs.end()) return fail(HEGJ_ERR_INVALID_ARG, "unknown ticket");
-    Request *r = it->second.get();
-    S->cv_done.wait(lk, [&] { return r->state == kDone || r->state == kFailed; });
+static hegj_status finished_info(const Request *r, hegj_request_info *info) {
     if (info) {
         std::memset(info, 0, sizeof(*info));
         info->ticket = r->ticket;
@@ -433,6 +421,26 @@ hegj_status hegj_serve_wait(hegj_server *S, uint64_t ticket, hegj_request_info *
     }
     if (r->state == kFailed) return fail(r->status, r->error);
     return HEGJ_OK;
+}
+
+hegj_status hegj_serve_wait(hegj_server *S, uint64_t ticket, hegj_request_info *info) {
+    if (!S) return fail(HEGJ_ERR_INVALID_ARG, "NULL server");
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(ticket);
+    if (it == S->reqs.end()) return fail(HEGJ_ERR_INVALID_ARG, "unknown ticket");
+    Request *r = it->second.get();
+    S->cv_done.wait(lk, [&] { return r->state == kDone || r->state == kFailed; });
+    return finished_info(r, info);
+}
+
+hegj_status hegj_serve_poll(hegj_server *S, uint64_t ticket, hegj_request_info *info) {
+    if (!S) return fail(HEGJ_ERR_INVALID_ARG, "NULL server");
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(ticket);
+    if (it == S->reqs.end()) return fail(HEGJ_ERR_INVALID_ARG, "unknown ticket");
+    const Request *r = it->second.get();
+    if (r->state != kDone && r->state != kFailed) return HEGJ_ERR_BUSY;
+    return finished_info(r, info);
 }
 
 hegj_status hegj_serve_copy(hegj_server *S, uint64_t ticket, uint64_t *offsets, uint32_t *neighbors) {

@@ -240,12 +240,16 @@ hegj_status hegj_result_copy_csr(const hegj_result *res, uint64_t *offsets, uint
  * the smallest cell side c with eps <= c * (1 - 2^-30).  `slots` requests are
  * on the device at once, each on its own stream: requests that arrive while
  * others run start as soon as a slot frees up (continuous batching).  Each
- * request first counts its result exactly, then waits (first come, first
- * served) until the result-buffer pool of `result_bytes` has room for it;
- * results stay in the pool until released.  A request whose result alone is
- * larger than the pool fails with HEGJ_ERR_CAPACITY.
+ * request first counts its result exactly, then is admitted into the
+ * result-buffer pool of `result_bytes` when its result fits (requests that
+ * wait for memory are served first come, first served; a request whose
+ * count finishes early does not wait for earlier ones still counting).
+ * Results stay in the pool until released, so a client should release
+ * finished results (hegj_serve_poll) rather than hold them while waiting for
+ * another request.  A request whose result alone is larger than the pool
+ * fails with HEGJ_ERR_CAPACITY.
  * ------------------------------------------------------------------------ */
-#define HEGJ_SERVE_MAX_LEVELS 8
+#define HEGJ_SERVE_MAX_LEVELS 16
 typedef struct hegj_server hegj_server;
 
 typedef struct {
@@ -292,6 +296,9 @@ hegj_status hegj_serve_submit(hegj_server *srv, const double *queries, uint32_t 
                               uint64_t *ticket);
 /* Block until the request has finished (or failed); fills *info. */
 hegj_status hegj_serve_wait(hegj_server *srv, uint64_t ticket, hegj_request_info *info);
+/* Non-blocking: HEGJ_OK and *info when the request has finished,
+ * HEGJ_ERR_BUSY while it runs, its status when it failed. */
+hegj_status hegj_serve_poll(hegj_server *srv, uint64_t ticket, hegj_request_info *info);
 /* Copy a finished request's CSR: offsets[nq+1], neighbors[pairs]. */
 hegj_status hegj_serve_copy(hegj_server *srv, uint64_t ticket, uint64_t *offsets, uint32_t *neighbors);
 /* Return the request's pool memory and forget it. */

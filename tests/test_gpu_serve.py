@@ -70,9 +70,25 @@ def test_alone_and_in_a_mix_identical(H, oracle, base):
         assert np.array_equal(off, a_off) and np.array_equal(nb, a_nb)
 
 
+def collect_as_finished(S, tickets, timeout_s=120):
+    """Release results in completion order (the serving contract: a client
+    does not hold finished results while it waits for another request)."""
+    import time
+    out, left, t0 = {}, list(tickets), time.time()
+    while left:
+        assert time.time() - t0 < timeout_s, "requests did not finish"
+        for t in list(left):
+            info = S.poll(t)
+            if info is not None:
+                out[t] = S.result(t)
+                left.remove(t)
+        time.sleep(0.001)
+    return out
+
+
 def test_admission_stays_within_the_pool(H, oracle, base):
     """A pool that holds about two results: every request completes (the
-    client releases in ticket order), the pool's high-water mark never
+    client releases results as they finish), the pool's high-water mark never
     exceeds it, and a request larger than the pool fails alone."""
     q = make_queries(oracle, 2500, 7)
     eoff, enb = oracle.range_join(base, q, 0.03)
@@ -85,13 +101,33 @@ def test_admission_stays_within_the_pool(H, oracle, base):
             S.wait(big)
         assert e.value.status == 6  # HEGJ_ERR_CAPACITY
         S.release(big)
-        for t in tickets:
-            off, nb = S.result(t)
+        for off, nb in collect_as_finished(S, tickets).values():
             assert np.array_equal(off, eoff)
             assert np.array_equal(sorted_rows(off, nb), enb)
         st = S.stats()
         assert st["pool_peak"] <= cap
         assert st["completed"] == 8 and st["failed"] == 1
+
+
+def test_poll_reports_finished_requests(H, oracle, base):
+    """poll: the info of a finished request (same as wait), an error for an
+    unknown ticket; results collected in completion order are complete."""
+    qs = [(make_queries(oracle, nq, seed), eps) for nq, eps, seed in REQS]
+    with H.Server(base, SIDES, slots=4, result_bytes=256 << 20, max_queries=5000) as S:
+        tickets = [S.submit(q, eps) for q, eps in qs]
+        got = collect_as_finished(S, tickets)
+        assert sorted(got) == sorted(tickets)
+        t = S.submit(qs[0][0], qs[0][1])
+        w = S.wait(t)
+        p = S.poll(t)
+        assert p["pairs"] == w["pairs"] and p["pair_hash"] == w["pair_hash"]
+        S.release(t)
+        with pytest.raises(H.HegjError):
+            S.poll(t)
+    for (q, eps), t in zip(qs, tickets):
+        off, nb = got[t]
+        eoff, enb = oracle.range_join(base, q, eps)
+        assert np.array_equal(off, eoff) and np.array_equal(sorted_rows(off, nb), enb)
 
 
 def test_concurrent_clients(H, oracle, base):
