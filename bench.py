@@ -47,6 +47,7 @@ SERVE = dict(dim=6, n=10_000_000, k=6, nq_lo=1_000, nq_hi=500_000, eps_lo=0.002,
              name="serving: 6-D 10M expo FP64 resident index; Poisson arrivals; request queries "
                   "log-uniform 1k-500k, eps log-uniform [0.002, 0.02]")
 PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector
 PEAK_BF16_TFLOPS = 2500.0 # MI355X dense bf16 MFMA
 PEAK_HBM_GBS = 8000.0     # MI355X HBM3E
 
@@ -114,6 +115,11 @@ def rooflines(n, dim, cells, pairs, evals, kernel_ms, refine):
         peak = PEAK_BF16_TFLOPS / 3.0
         note = ("2 D flops per candidate pair; peak = bf16 dense MFMA / 3 (split-BF16 "
                 "hi.hi + hi.lo + lo.hi); kernel k_gb_join")
+    elif refine == "f32":
+        flops = 3.0 * dim * evals
+        peak = PEAK_FP32_TFLOPS
+        note = ("3 D FP32 flops (sub + fma per dim) per reference candidate, FP32 vector peak; "
+                "kernel k_join with the FP32 filter")
     else:
         flops = 3.0 * dim * evals
         peak = PEAK_FP64_TFLOPS
@@ -128,7 +134,7 @@ def attach_traffic(roof, config, refine, kernel_ms):
     """HBM traffic per launch of the dominant kernel, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same workload (each
     counter scaled by its reading of the bench's 1 GiB copy kernel)."""
-    tag = config + ("_mfma" if refine == "mfma" else "")
+    tag = config + {"mfma": "_mfma", "f32": "_f32"}.get(refine, "")
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return
@@ -261,7 +267,7 @@ def main():
     ap.add_argument("--xcd-parts", type=int, default=None)
     ap.add_argument("--parts-by-block", type=int, default=0)
     ap.add_argument("--grab", type=int, default=None)
-    ap.add_argument("--refine", default=None, choices=["auto", "fp64", "mfma"])
+    ap.add_argument("--refine", default=None, choices=["auto", "fp64", "mfma", "f32"])
     ap.add_argument("--graph", action="store_true",
                     help="replay the captured batch graph (FP64 refine)")
     ap.add_argument("--n", type=int, default=0, help="override the point count (experiments)")
@@ -332,7 +338,8 @@ def main():
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": ms_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if refine_used != "mfma" else "bf16 hi/lo (f32 acc) + f64 band recheck", "data": "synthetic (reference exponential generator, seed 0)",
+        "dtype": {"mfma": "bf16 hi/lo (f32 acc) + f64 band recheck",
+                  "f32": "f32 filter + f64 band recheck"}.get(refine_used, "f64"), "data": "synthetic (reference exponential generator, seed 0)",
         "config": {"workload": cfg["name"], "n": n, "dim": dim, "indexed_dims": k,
                    "eps": eps, "pairs": pairs, "total_candidates": total_cand,
                    "cells": iinfo["nNonEmptyCells"], "cell0_silenced": iinfo["n_cell0_queries"],

@@ -105,7 +105,7 @@ def _struct_dict(s):
         if name == "reserved":
             continue
         if name == "refine_used":
-            v = {1: "fp64", 2: "mfma"}.get(v, "none")
+            v = {1: "fp64", 2: "mfma", 3: "f32"}.get(v, "none")
         if isinstance(v, C.Array):
             v = list(v)
         out[name] = v
@@ -135,6 +135,8 @@ _lib.hegj_result_free.argtypes = [_vp]
 _lib.hegj_result_get_info.argtypes = [_vp, C.POINTER(_ResultInfo)]
 _lib.hegj_result_copy_counts.argtypes = [_vp, _vp]
 _lib.hegj_result_copy_csr.argtypes = [_vp, _vp, _vp, C.c_int]
+_lib.hegj_f32_thresholds.argtypes = [C.c_double, C.c_uint32, C.c_double, C.POINTER(C.c_float),
+                                     C.POINTER(C.c_float)]
 _lib.hegj_join_opts_tuned.argtypes = [C.POINTER(_JoinOpts), C.c_uint32]
 _lib.hegj_tuning_set.argtypes = [C.c_uint32, C.POINTER(_JoinOpts)]
 _lib.hegj_tuning_load.argtypes = [C.c_char_p, C.POINTER(C.c_uint32)]
@@ -165,6 +167,13 @@ def load_tuning(path=TUNING_PATH):
 
 if os.path.exists(TUNING_PATH):
     load_tuning()
+
+
+def f32_thresholds(epsilon, dim, radius):
+    """(tacc, trej) of the FP32 filter (see hegj_f32_thresholds)."""
+    a, b = C.c_float(), C.c_float()
+    _check(_lib.hegj_f32_thresholds(float(epsilon), int(dim), float(radius), C.byref(a), C.byref(b)))
+    return a.value, b.value
 
 
 def version() -> str:
@@ -300,7 +309,8 @@ class GridIndex:
         unit_order: "morton", "spatial" (linear cell id) or "workload"
         (descending candidates, the reference's order); refine: "auto",
         "fp64" or "mfma" (split-BF16 MFMA + guard band + FP64 band
-        recheck); xcd_parts: unit ranges, one per XCD (0 = auto);
+        recheck) or "f32" (FP32 distance filter + exact FP64 recheck of its
+        band); xcd_parts: unit ranges, one per XCD (0 = auto);
         use_graph: keep the FP64 plan in the index and replay a captured
         hipGraph of the batch (repeated joins with the same options)."""
         o = _JoinOpts()
@@ -313,7 +323,7 @@ class GridIndex:
         if xcd_parts is not None:
             o.xcd_parts = xcd_parts
         if refine is not None:
-            o.refine = {"auto": 0, "fp64": 1, "mfma": 2}[refine]
+            o.refine = {"auto": 0, "fp64": 1, "mfma": 2, "f32": 3}[refine]
         if units_per_grab is not None:
             o.units_per_grab = units_per_grab
         o.part_by_block = part_by_block

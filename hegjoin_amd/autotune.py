@@ -27,7 +27,7 @@ import sys
 import time
 
 ORDERS = {"spatial": 0, "workload": 1, "morton": 2}
-REFINES = {"auto": 0, "fp64": 1, "mfma": 2}
+REFINES = {"auto": 0, "fp64": 1, "mfma": 2, "f32": 3}
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
@@ -45,9 +45,12 @@ def candidates(dim, with_fp64=False):
                 out.append(dict(refine="fp64", unit_order="morton", queries_per_unit=qb,
                                 xcd_parts=8, units_per_grab=0))
         return out
-    for qb, order, parts, grab in itertools.product((4, 8, 16), ("morton", "workload", "spatial"),
-                                                    (1, 8), (0, 1, 4)):
-        out.append(dict(refine="fp64", unit_order=order, queries_per_unit=qb, xcd_parts=parts,
+    refines = ("fp64", "f32") if dim in (2, 6) else ("fp64",)
+    for refine, qb, order, parts, grab in itertools.product(
+            refines, (4, 8, 16), ("morton", "workload", "spatial"), (1, 8), (0, 1, 4)):
+        if refine == "f32" and qb * dim > 48:
+            continue  # not instantiated
+        out.append(dict(refine=refine, unit_order=order, queries_per_unit=qb, xcd_parts=parts,
                         units_per_grab=grab))
     return out
 

@@ -113,6 +113,9 @@ struct hegj_index {
     double gb_ms = 0;
     // FP64 join plan + captured batch graph, reused while the options match
     hegj::GraphPlan *gplan = nullptr;
+    // centred FP32 copy for the FP32 filter (lazily computed)
+    float *soa32 = nullptr;
+    double f32_radius = 0;
 };
 
 struct hegj_result {
@@ -209,12 +212,53 @@ struct Counters {
     unsigned long long hash;
     unsigned long long evals;
     unsigned long long pairs;
+    unsigned long long band;  // FP32 filter: pairs rechecked in FP64
 };
 
 hegj_status ensure_morton(hegj_index *h) {
     if (h->d_morton) return HEGJ_OK;
     HIP_TRY(hipMalloc(&h->d_morton, sizeof(uint32_t) * (h->ix.C + 1)));
     return morton_order(h->g, h->ix, h->d_morton, &h->morton_ok, h->stream);
+}
+
+// Bound on |acc32 - acc| for a pair whose exact squared distance is S: acc32
+// is the FP32 fma chain over d^ = fl32(x^_c - x^_q), x^ = fl32(fl64(x - c))
+// with |x - c| <= R, acc the reference's FP64 fma chain over fl64(x_c - x_q).
+//   |x^ - (x - c)| <= a R,  a = u + v + uv          (u = 2^-24, v = 2^-53)
+//   |d^_l - d_l| <= A + u |d_l|,  A = 2 a R (1 + u)
+//   |sum d^^2 - S| <= 2.0001 A sqrt(D S) + D A^2 + 2.0001 u S   (Cauchy-Schwarz)
+//   |acc32 - sum d^^2| <= 1.0001 D u sum d^^2        (D roundings of nonnegative sums)
+//   |acc - S| <= 1.0001 (D + 2) v S
+// plus D * 2^-126 for FP32 values flushed to zero.
+double f32_bound(double S, double D, double R) {
+    const double u = std::ldexp(1.0, -24), v = std::ldexp(1.0, -53);
+    const double a = u + v + u * v;
+    const double A = 2.0 * a * R * (1.0 + u);
+    const double sumdiff = 2.0001 * A * std::sqrt(D * S) + D * A * A + 2.0001 * u * S;
+    return (1.0 + 1.0001 * D * u) * sumdiff + 1.0001 * D * u * S + 1.0001 * (D + 2.0) * v * S +
+           D * std::ldexp(1.0, -126);
+}
+
+// Thresholds of the FP32 filter: acc32 <= tacc => acc <= T, acc32 > trej =>
+// acc > T (twice the bound at S = 1.001 T, which covers every S whose acc or
+// acc32 can reach the thresholds).  False when the band would be wider than
+// 1% of T (then the filter would recheck too much to pay off).
+bool f32_thresholds(double T, uint32_t dim, double R, float *tacc, float *trej) {
+    if (!(T > 1e-30) || !(R < 1e30)) return false;
+    const double B = 2.0 * f32_bound(1.001 * T, (double)dim, R);
+    if (!(B <= 0.01 * T)) return false;
+    const double lo = T - B, hi = T + B;
+    float fl = (float)lo, fh = (float)hi;
+    if ((double)fl > lo) fl = std::nextafter(fl, -std::numeric_limits<float>::infinity());
+    if ((double)fh < hi) fh = std::nextafter(fh, std::numeric_limits<float>::infinity());
+    *tacc = fl;
+    *trej = fh;
+    return true;
+}
+
+hegj_status ensure_soa32(hegj_index *h) {
+    if (h->soa32) return HEGJ_OK;
+    return make_soa32(h->g, h->ix, &h->soa32, &h->f32_radius, h->stream);
 }
 
 // Everything a FP64 join needs besides its result: work units in the chosen
@@ -233,6 +277,8 @@ struct JoinPlan {
     uint32_t part_end[kMaxParts] = {};
     uint64_t capacity = 0;
     double estimate_ms = 0;
+    bool f32 = false;  // FP32 filter with FP64 band recheck
+    float tacc32 = 0, trej32 = 0;
     JoinPlan() = default;
     JoinPlan(const JoinPlan &) = delete;
     JoinPlan &operator=(const JoinPlan &) = delete;
@@ -277,12 +323,31 @@ JoinArgs base_args(const hegj_index *h, const hegj_join_opts &opts, const JoinPl
     a.gmask = P.gmask;
     a.gmask_slots = P.gmask_slots;
     a.part_by_xcc = opts.part_by_block ? 0u : 1u;
+    a.band = &P.d_ctr->band;
+    if (P.f32) {
+        a.f32 = 1;
+        a.soa32 = h->soa32;
+        a.tacc32 = P.tacc32;
+        a.trej32 = P.trej32;
+    }
     return a;
 }
 
 hegj_status make_plan(hegj_index *h, const hegj_join_opts &opts, uint32_t qb, const uint32_t *order,
                       JoinPlan &P) {
     hipStream_t s = h->stream;
+    if (opts.refine == HEGJ_REFINE_F32 || opts.refine == HEGJ_REFINE_AUTO) {
+        bool ok = f32_supported(h->g.dim, qb);
+        if (ok) {
+            HEGJ_TRY(ensure_soa32(h));
+            ok = f32_thresholds(h->g.thresh, h->g.dim, h->f32_radius, &P.tacc32, &P.trej32);
+        }
+        if (!ok && opts.refine == HEGJ_REFINE_F32)
+            return fail(HEGJ_ERR_UNSUPPORTED,
+                        "FP32 filter: built for dim 2 (4/8/16 queries per unit) and 6 (4/8), "
+                        "and needs a band below 1% of eps^2");
+        P.f32 = ok;
+    }
     HEGJ_TRY(build_units(h->g, h->ix, qb, order, &P.units, &P.U, s));
     const uint32_t U = P.U;
     HIP_TRY(hipMalloc(&P.units_next, sizeof(uint2) * (U + 1)));
@@ -426,6 +491,8 @@ hegj_status run_batches(hegj_index *h, const hegj_join_opts &opts, uint32_t qb, 
     r->info.total_neighbors = c.pairs;
     r->info.pair_hash = c.hash;
     r->info.evals_executed = c.evals;
+    r->info.band_rechecks = c.band;
+    r->info.refine_used = P.f32 ? HEGJ_REFINE_F32 : HEGJ_REFINE_FP64;
     r->info.n_batches = (uint32_t)r->batches.size();
     r->info.join_ms = join_ms;
     r->info.join_launches = launches;
@@ -461,7 +528,8 @@ namespace {
 hegj_status capture_graph(hegj_index *h, GraphPlan &G) {
     hipStream_t s = h->stream;
     JoinArgs proto{};
-    proto.g = h->g;  // selects the kernel instantiation
+    proto.g = h->g;  // with f32: selects the kernel instantiation
+    proto.f32 = G.P.f32 ? 1u : 0u;
     hipGraph_t graph = nullptr;
     HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hegj_status st = HEGJ_OK;
@@ -490,7 +558,7 @@ hegj_status selfjoin_graph(hegj_index *h, const hegj_join_opts &opts, uint32_t q
     const uint32_t n = h->g.n;
     const uint32_t key[8] = {qb, opts.unit_order, opts.xcd_parts, opts.units_per_grab,
                              opts.part_by_block, (uint32_t)opts.result_capacity,
-                             (uint32_t)(opts.result_capacity >> 32), 1u};
+                             (uint32_t)(opts.result_capacity >> 32), opts.refine};
     if (h->gplan && std::memcmp(h->gplan->key, key, sizeof(key)) != 0) {
         free_graph_plan(h->gplan);
         h->gplan = nullptr;
@@ -549,6 +617,8 @@ hegj_status selfjoin_graph(hegj_index *h, const hegj_join_opts &opts, uint32_t q
     r->info.total_neighbors = c.pairs;
     r->info.pair_hash = c.hash;
     r->info.evals_executed = c.evals;
+    r->info.band_rechecks = c.band;
+    r->info.refine_used = G.P.f32 ? HEGJ_REFINE_F32 : HEGJ_REFINE_FP64;
     r->info.n_batches = 1;
     r->info.join_launches = 1;
     r->info.join_kernel_avg_ms = r->info.join_ms;
@@ -697,6 +767,7 @@ hegj_status hegj_index_free(hegj_index *h) {
     if (h->gb_B) (void)hipFree(h->gb_B);
     if (h->gb_w) (void)hipFree(h->gb_w);
     if (h->gplan) free_graph_plan(h->gplan);
+    if (h->soa32) (void)hipFree(h->soa32);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return HEGJ_OK;
@@ -807,7 +878,7 @@ hegj_status hegj_tuning_set(uint32_t dim, const hegj_join_opts *opts) {
     if (opts->queries_per_unit != 0 && opts->queries_per_unit != 4 && opts->queries_per_unit != 8 &&
         opts->queries_per_unit != 16)
         return fail(HEGJ_ERR_INVALID_ARG, "queries_per_unit must be 0, 4, 8 or 16");
-    if (opts->unit_order > HEGJ_ORDER_MORTON || opts->refine > HEGJ_REFINE_MFMA)
+    if (opts->unit_order > HEGJ_ORDER_MORTON || opts->refine > HEGJ_REFINE_F32)
         return fail(HEGJ_ERR_INVALID_ARG, "unknown unit_order or refine");
     std::lock_guard<std::mutex> lk(g_tuning_mu);
     g_tuning[dim] = *opts;
@@ -857,7 +928,7 @@ hegj_status hegj_selfjoin(hegj_index *h, const hegj_join_opts *opts_in, hegj_res
         opts = *opts_in;
     else
         hegj_join_opts_tuned(&opts, h->g.dim);
-    if (opts.refine > HEGJ_REFINE_MFMA) return fail(HEGJ_ERR_INVALID_ARG, "unknown refine");
+    if (opts.refine > HEGJ_REFINE_F32) return fail(HEGJ_ERR_INVALID_ARG, "unknown refine");
     const bool mfma = opts.refine == HEGJ_REFINE_MFMA ||
                       (opts.refine == HEGJ_REFINE_AUTO && h->g.dim >= 16 && gb_kblocks(h->g.dim) != 0);
     if (mfma && gb_kblocks(h->g.dim) == 0)
@@ -914,6 +985,15 @@ hegj_status hegj_selfjoin(hegj_index *h, const hegj_join_opts *opts_in, hegj_res
     r->info.total_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     *out = r.release();
+    return HEGJ_OK;
+}
+
+hegj_status hegj_f32_thresholds(double epsilon, uint32_t dim, double radius, float *tacc,
+                                float *trej) {
+    if (!tacc || !trej || dim == 0 || !(epsilon > 0.0) || !(radius >= 0.0))
+        return fail(HEGJ_ERR_INVALID_ARG, "bad argument");
+    if (!f32_thresholds(sqrt_threshold(epsilon), dim, radius, tacc, trej))
+        return fail(HEGJ_ERR_UNSUPPORTED, "FP32 band wider than 1% of eps^2");
     return HEGJ_OK;
 }
 

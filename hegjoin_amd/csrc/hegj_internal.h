@@ -133,6 +133,12 @@ struct JoinArgs {
     unsigned long long *evals;    // distance evaluations performed
     unsigned long long *pairs;    // pairs found (count-only) or written
     uint32_t count_only;
+    // FP32 filter (f32 != 0): centred float coordinates and the two
+    // thresholds that make acc32 decide exactly outside the band
+    uint32_t f32;
+    const float *soa32;           // [dim][n], cell order
+    float tacc32, trej32;
+    unsigned long long *band;     // pairs rechecked in FP64
 };
 
 // ---- index (hegj_index.hip) ----------------------------------------------
@@ -148,6 +154,12 @@ hegj_status build_units(const GridGeom &g, const DevIndex &ix, uint32_t qb, cons
 // `a` then only selects the instantiation by a.g.dim).
 hegj_status launch_join(const JoinArgs &a, uint32_t qb, uint32_t blocks, hipStream_t s,
                         const JoinArgs *dev_args = nullptr);
+// FP32 filter support: centred float copy of the cell-ordered coordinates
+// and the largest |x - centre| (hegj_util.hip).
+hegj_status make_soa32(const GridGeom &g, const DevIndex &ix, float **soa32, double *radius,
+                       hipStream_t s);
+// Whether the FP32 filter is instantiated for (dim, qb).
+bool f32_supported(uint32_t dim, uint32_t qb);
 // Zero `bytes` (a multiple of 4) with a kernel (safe inside graph capture).
 hegj_status zero_words(void *p, size_t bytes, hipStream_t s);
 // Workgroups of the persistent join kernel that fit on the device at once.

@@ -251,10 +251,14 @@ constexpr int join_min_waves() {
     return (D > 0 && D <= 8 && QB <= 8) ? 4 : 1;
 }
 
-template <int D, int QB, bool IND = false>
+// F32: candidates and queries come from the centred FP32 copy; a pair is
+// accepted when acc32 <= tacc32, rejected when acc32 > trej32, and rechecked
+// with the exact FP64 chain in between (the thresholds bound |acc32 - acc|).
+template <int D, int QB, bool IND = false, bool F32 = false>
 __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB>())) void k_join(
     JoinArgs av, const JoinArgs *__restrict__ ap) {
     const JoinArgs &a = IND ? *ap : av;
+    static_assert(!F32 || (D > 0 && D <= 8 && QB * D <= 48), "FP32 filter: small D only");
     __shared__ RowList rls[kWavesPerBlock];
     __shared__ HitStore hs_all[kWavesPerBlock];
     extern __shared__ double q_dyn[];  // QB * dim query coordinates per wave
@@ -269,7 +273,9 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB>())) voi
     constexpr bool kQS = q_in_sgprs<D, QB>();
     const uint32_t dimq = D > 0 ? (uint32_t)D : a.g.dim;
     double *const qs = q_dyn + (size_t)wid * QB * dimq;
-    double qreg[kQS ? QB : 1][kQS ? DR : 1];
+    double qreg[kQS && !F32 ? QB : 1][kQS && !F32 ? DR : 1];
+    float qregf[F32 ? QB : 1][F32 ? DR : 1];
+    uint64_t my_band = 0;
     uint32_t part = (a.part_by_xcc ? xcc_id() : blockIdx.x) % a.n_parts;
     uint32_t tries = 0, grab = 0, gend = 0;
 
@@ -285,7 +291,16 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB>())) voi
             bool live;
             const uint32_t M = build_rows(a, cell, rl, &nslots, &live);
             // the unit's query coordinates, once: lane i loads element i
-            if (kQS) {
+            if constexpr (F32) {
+                constexpr int NQD = QB * DR;
+                const int qi = lane / DR, l = lane % DR;
+                float v = 0.0f;
+                if (lane < NQD && (uint32_t)qi < nq) v = a.soa32[(uint64_t)l * a.g.n + qpos0 + qi];
+#pragma unroll
+                for (int j = 0; j < NQD; j++)
+                    qregf[j / DR][j % DR] =
+                        __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(v), j));
+            } else if (kQS) {
                 constexpr int NQD = kQS ? QB * DR : 1;
 #pragma unroll
                 for (int i0 = 0; i0 < NQD; i0 += 64) {
@@ -299,7 +314,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB>())) voi
                         const uint64_t u = (uint64_t)__double_as_longlong(v);
                         const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, j - i0);
                         const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), j - i0);
-                        qreg[kQS ? j / DR : 0][kQS ? j % DR : 0] =
+                        qreg[kQS && !F32 ? j / DR : 0][kQS && !F32 ? j % DR : 0] =
                             __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
                     }
                 }
@@ -321,8 +336,19 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB>())) voi
             // register ring (the loop is unrolled by kDepth so ring indices
             // are static): several chunks of loads stay in flight per wave.
             constexpr int kDepth = kRegs ? (D <= 8 ? 4 : 2) : 1;
-            double ring[kDepth][DR];
-            if (kRegs) {
+            double ring[F32 ? 1 : kDepth][DR];
+            float ringf[F32 ? kDepth : 1][DR];
+            uint32_t posr[F32 ? kDepth : 1];
+            if constexpr (F32) {
+#pragma unroll
+                for (int p = 0; p < kDepth; p++) {
+                    const uint32_t t = p * 64 + lane;
+                    const bool v = t < M;
+                    posr[p] = v ? cand_pos_adv(rl, nslots, t, slot) : 0u;
+#pragma unroll
+                    for (int l = 0; l < DR; l++) ringf[p][l] = a.soa32[(uint64_t)l * a.g.n + posr[p]];
+                }
+            } else if (kRegs) {
 #pragma unroll
                 for (int p = 0; p < kDepth; p++) {
                     const uint32_t t = p * 64 + lane;
@@ -342,7 +368,50 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB>())) voi
                     if (!kRegs && ch >= nchunks) break;
                     double acc[QB];
                     const bool valid = ch * 64 + lane < M;
-                    if (kRegs) {
+                    if constexpr (F32) {
+                        float accf[QB];
+#pragma unroll
+                        for (int qi = 0; qi < QB; qi++) {
+                            accf[qi] = 0.0f;
+#pragma unroll
+                            for (int l = 0; l < DR; l++) {
+                                const float dd = ringf[p][l] - qregf[qi][l];
+                                accf[qi] = __builtin_fmaf(dd, dd, accf[qi]);
+                            }
+                        }
+                        const uint32_t pos = posr[p];
+                        {
+                            const uint32_t tn = (ch + kDepth) * 64 + lane;
+                            const bool vn = tn < M;
+                            posr[p] = vn ? cand_pos_adv(rl, nslots, tn, slot) : 0u;
+#pragma unroll
+                            for (int l = 0; l < DR; l++) ringf[p][l] = a.soa32[(uint64_t)l * a.g.n + posr[p]];
+                        }
+                        uint64_t mine = 0;
+#pragma unroll
+                        for (int qi = 0; qi < QB; qi++) {
+                            const bool live_q = valid && (uint32_t)qi < nq;
+                            uint64_t m = __ballot(live_q && accf[qi] <= a.tacc32);
+                            const bool band = live_q && !(accf[qi] <= a.tacc32) && !(accf[qi] > a.trej32);
+                            const uint64_t mb = __ballot(band);
+                            if (mb) {
+                                // the band: the reference's exact FP64 chain
+                                double e = 0.0;
+#pragma unroll
+                                for (int l = 0; l < DR; l++) {
+                                    const double dd = a.ix.soa[(uint64_t)l * a.g.n + pos] -
+                                                      a.ix.soa[(uint64_t)l * a.g.n + qpos0 + qi];
+                                    e = __builtin_fma(dd, dd, e);
+                                }
+                                m |= __ballot(band && e <= T);
+                                my_band += (uint64_t)__popcll(mb);
+                            }
+                            cnt_l += lane == qi ? (uint32_t)__popcll(m) : 0u;
+                            mine = lane == qi ? m : mine;
+                        }
+                        if (keep && (uint32_t)lane < nq && ch < nchunks) hs[lane * nchunks + ch] = mine;
+                        continue;
+                    } else if (kRegs) {
                         // use ring[p], then refill it with chunk ch + kDepth
                         // (no copy of registers that still have loads in flight)
                         if constexpr (kQS) {
@@ -438,7 +507,9 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB>())) voi
                         if constexpr (kRegs) {
 #pragma unroll
                             for (int l = 0; l < DR; l++) {
-                                const double q = kQS ? qreg[kQS ? qi : 0][kQS ? l : 0] : qs[qi * DR + l];
+                                const double q = F32 ? a.ix.soa[(uint64_t)l * a.g.n + qpos0 + qi]
+                                                 : kQS ? qreg[kQS && !F32 ? qi : 0][kQS && !F32 ? l : 0]
+                                                       : qs[qi * DR + l];
                                 const double dd = x[l] - q;
                                 acc1 = __builtin_fma(dd, dd, acc1);
                             }
@@ -463,6 +534,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB>())) voi
     // one atomic per wave for the counters
     for (int o = 32; o > 0; o >>= 1) my_hash += __shfl_xor(my_hash, o, 64);
     if (lane == 0) {
+        if (F32 && my_band && a.band) atomicAdd(a.band, (unsigned long long)my_band);
         if (my_hash) atomicAdd(a.hash, (unsigned long long)my_hash);
         if (my_evals) atomicAdd(a.evals, (unsigned long long)my_evals);
         if (my_pairs) atomicAdd(a.pairs, (unsigned long long)my_pairs);
@@ -552,15 +624,30 @@ hegj_status build_units(const GridGeom &g, const DevIndex &ix, uint32_t qb, cons
 }
 
 template <int D, int QB>
+constexpr bool f32_inst() {
+    return (D == 2 || D == 6) && QB * D <= 48;
+}
+
+template <int D, int QB>
 static hegj_status launch_join_t(const JoinArgs &a, const JoinArgs *ap, uint32_t blocks,
                                  hipStream_t s) {
     const uint32_t lds = q_lds_bytes<D, QB>(a.g.dim);
     if (lds > kMaxQueryLds)
         return fail(HEGJ_ERR_UNSUPPORTED, "queries_per_unit * dim too large for LDS; lower queries_per_unit");
-    if (ap)
+    if (a.f32) {
+        if constexpr (f32_inst<D, QB>()) {
+            if (ap)
+                k_join<D, QB, true, true><<<blocks, 64 * kWavesPerBlock, lds, s>>>(JoinArgs{}, ap);
+            else
+                k_join<D, QB, false, true><<<blocks, 64 * kWavesPerBlock, lds, s>>>(a, nullptr);
+        } else {
+            return fail(HEGJ_ERR_UNSUPPORTED, "no FP32-filter kernel for this dim / queries_per_unit");
+        }
+    } else if (ap) {
         k_join<D, QB, true><<<blocks, 64 * kWavesPerBlock, lds, s>>>(JoinArgs{}, ap);
-    else
+    } else {
         k_join<D, QB, false><<<blocks, 64 * kWavesPerBlock, lds, s>>>(a, nullptr);
+    }
     HIP_TRY(hipGetLastError());
     return HEGJ_OK;
 }
@@ -575,6 +662,10 @@ static hegj_status launch_join_qb(const JoinArgs &a, const JoinArgs *ap, uint32_
         case 90: return launch_join_t<90, QB>(a, ap, blocks, s);
         default: return launch_join_t<0, QB>(a, ap, blocks, s);
     }
+}
+
+bool f32_supported(uint32_t dim, uint32_t qb) {
+    return (dim == 2 && (qb == 4 || qb == 8 || qb == 16)) || (dim == 6 && (qb == 4 || qb == 8));
 }
 
 hegj_status launch_join(const JoinArgs &a, uint32_t qb, uint32_t blocks, hipStream_t s,

@@ -3,6 +3,7 @@
 // neighbour-table to CSR gather (the device form of
 // constructNeighborTableKeyValueWithPtrs, GPU.cu:1731-1776).
 #include <algorithm>
+#include <vector>
 
 #include "hegj_internal.h"
 
@@ -327,6 +328,80 @@ hegj_status split_units(const DevIndex &ix, const uint2 *units, uint32_t n_units
     k_find_splits<<<1, 64, 0, s>>>(prefix, n_units, total, n_parts, d_end);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(part_end, d_end, sizeof(uint32_t) * n_parts, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return HEGJ_OK;
+}
+
+namespace {
+
+// Per-dimension min / max of the cell-ordered coordinates, one partial per block.
+__global__ __launch_bounds__(256) void k_dim_minmax(const double *__restrict__ soa, uint32_t n,
+                                                    uint32_t dim, double *__restrict__ part) {
+    __shared__ double smin[256], smax[256];
+    for (uint32_t l = 0; l < dim; l++) {
+        double lo = 1.0 / 0.0, hi = -1.0 / 0.0;
+        for (uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x; p < n; p += (uint64_t)gridDim.x * 256) {
+            const double v = soa[(uint64_t)l * n + p];
+            lo = fmin(lo, v);
+            hi = fmax(hi, v);
+        }
+        smin[threadIdx.x] = lo;
+        smax[threadIdx.x] = hi;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) {
+                smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + o]);
+                smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + o]);
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            part[((uint64_t)blockIdx.x * dim + l) * 2] = smin[0];
+            part[((uint64_t)blockIdx.x * dim + l) * 2 + 1] = smax[0];
+        }
+        __syncthreads();
+    }
+}
+
+// soa32 = (float)(soa - centre): the FP64 difference rounded to FP32.
+__global__ __launch_bounds__(256) void k_soa32(const double *__restrict__ soa, uint64_t total,
+                                               uint32_t n, const double *__restrict__ centre,
+                                               float *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < total) out[i] = (float)(soa[i] - centre[i / n]);
+}
+
+}  // namespace
+
+hegj_status make_soa32(const GridGeom &g, const DevIndex &ix, float **soa32, double *radius,
+                       hipStream_t s) {
+    const uint32_t n = g.n, dim = g.dim;
+    const uint32_t blocks = std::min<uint32_t>(ceil_div(n, 256), 512);
+    Scratch sc;
+    double *part, *centre;
+    HIP_TRY(sc.get(&part, (size_t)blocks * dim * 2));
+    HIP_TRY(sc.get(&centre, dim));
+    k_dim_minmax<<<blocks, 256, 0, s>>>(ix.soa, n, dim, part);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> hp((size_t)blocks * dim * 2), c(dim);
+    HIP_TRY(hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    double R = 0.0;
+    for (uint32_t l = 0; l < dim; l++) {
+        double lo = hp[l * 2], hi = hp[l * 2 + 1];
+        for (uint32_t b = 1; b < blocks; b++) {
+            lo = std::min(lo, hp[((size_t)b * dim + l) * 2]);
+            hi = std::max(hi, hp[((size_t)b * dim + l) * 2 + 1]);
+        }
+        c[l] = lo + (hi - lo) * 0.5;
+        R = std::max(R, std::max(hi - c[l], c[l] - lo));
+    }
+    *radius = R * (1.0 + 1e-12);
+    HIP_TRY(hipMemcpyAsync(centre, c.data(), sizeof(double) * dim, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMalloc(soa32, sizeof(float) * (size_t)n * dim));
+    const uint64_t total = (uint64_t)n * dim;
+    k_soa32<<<ceil_div(total, 256), 256, 0, s>>>(ix.soa, total, n, centre, *soa32);
+    HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s));
     return HEGJ_OK;
 }

@@ -165,9 +165,11 @@ typedef enum {
 } hegj_unit_order;
 
 typedef enum {
-    HEGJ_REFINE_AUTO = 0,     /* MFMA guard band when dim >= 16, else FP64 */
+    HEGJ_REFINE_AUTO = 0,     /* MFMA guard band when dim >= 16, FP32 filter where it is
+                                 built (dim 2 and 6) and its band is narrow, else FP64 */
     HEGJ_REFINE_FP64 = 1,     /* exact FP64 chain for every candidate */
-    HEGJ_REFINE_MFMA = 2      /* split-BF16 MFMA tiles + guard band + FP64 band recheck */
+    HEGJ_REFINE_MFMA = 2,     /* split-BF16 MFMA tiles + guard band + FP64 band recheck */
+    HEGJ_REFINE_F32 = 3       /* FP32 distance filter + exact FP64 recheck of its band */
 } hegj_refine;
 
 typedef struct {
@@ -195,6 +197,13 @@ hegj_status hegj_tuning_set(uint32_t dim, const hegj_join_opts *opts);
 hegj_status hegj_tuning_load(const char *path, uint32_t *n_loaded);
 
 hegj_status hegj_selfjoin(hegj_index *idx, const hegj_join_opts *opts, hegj_result **out);
+/* Thresholds of the FP32 filter for coordinates within `radius` of their
+ * centre: an FP32 fma-chain squared distance <= *tacc is within epsilon, one
+ * > *trej is not, by the reference's FP64 chain; pairs in between are
+ * rechecked in FP64.  HEGJ_ERR_UNSUPPORTED when the band would exceed 1% of
+ * epsilon^2.  (No reference counterpart; exposed for callers and tests.) */
+hegj_status hegj_f32_thresholds(double epsilon, uint32_t dim, double radius, float *tacc,
+                                float *trej);
 hegj_status hegj_result_free(hegj_result *res);
 
 typedef struct {

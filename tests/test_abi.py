@@ -239,3 +239,56 @@ def test_bench_rooflines_pick_the_binding_roof():
     roof, other = B.rooflines(2_000_000, 18, 2900, 7.4e7, 2.8e12, 500.0, "mfma")
     assert roof["bound"] == "mfma" and abs(roof["peak"] - 2500 / 3) < 1e-9
     assert abs(roof["frac"] - 2 * 18 * 2.8e12 / 0.5 / 1e12 / (2500 / 3)) < 1e-12
+
+
+def _fmaf(a, b, c):
+    """float32 fma: the exact product of two float32 values fits a double;
+    one rounding of product + c to float32 (c and the product are exact in
+    float64 up to a final rounding far below float32 precision)."""
+    return np.float32(np.float64(a) * np.float64(b) + np.float64(c))
+
+
+def test_f32_filter_thresholds_are_sound(oracle):
+    """The FP32 filter's decisions agree with the reference's FP64 chain for
+    every pair outside the band: pairs planted at eps(1 +- tiny) with
+    coordinates up to `radius` from the centre, in 2, 6 and 8 dims."""
+    import hegjoin_amd as H
+    rng = np.random.default_rng(5)
+    checked = 0
+    for dim, eps, R in [(2, 0.001, 0.5), (6, 0.0063, 0.5), (6, 0.002, 8.0), (8, 0.05, 1.0)]:
+        tacc, trej = H.f32_thresholds(eps, dim, R)
+        T = oracle.threshold(eps)
+        assert tacc < T < trej and (trej - tacc) < 0.02 * T
+        centre = rng.uniform(-3, 3, dim)
+        q = centre + rng.uniform(-R, R, (4000, dim)) * 0.999
+        d = rng.standard_normal((4000, dim))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        rel = rng.choice([-1e-3, -1e-5, -1e-7, 0.0, 1e-7, 1e-5, 1e-3], 4000)
+        c = q + d * (eps * (1 + rel))[:, None]
+        c = centre + np.clip(c - centre, -R, R)
+        qf = (q - centre).astype(np.float32)   # fl32(fl64(x - c)): numpy rounds to nearest
+        cf = (c - centre).astype(np.float32)
+        for i in range(len(q)):
+            acc32 = np.float32(0)
+            acc = 0.0
+            for l in range(dim):
+                df = np.float32(cf[i, l] - qf[i, l])
+                acc32 = _fmaf(df, df, acc32)
+                dd = c[i, l] - q[i, l]
+                # FP64 multiply-add: differs from the reference's fma chain by
+                # ~1e-15 relative, far inside the band (>= 1e-6 relative)
+                acc = dd * dd + acc
+            # outside the band the filter's decision must be the FP64 one
+            if acc32 <= tacc:
+                assert acc <= T, (dim, i)
+            elif acc32 > trej:
+                assert acc > T, (dim, i)
+            checked += 1
+    assert checked == 16000
+
+
+def test_f32_thresholds_reject_wide_bands():
+    import hegjoin_amd as H
+    with pytest.raises(H.HegjError) as e:
+        H.f32_thresholds(1e-6, 6, 1e3)  # tiny eps, huge coordinates
+    assert e.value.status == 2

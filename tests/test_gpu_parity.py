@@ -234,6 +234,73 @@ def test_mfma_large_offset_coordinates(H, oracle):
     assert np.array_equal(off, eoff) and np.array_equal(nb, enb)
 
 
+LOWD = [p for p in golden_files("alg0_") + golden_files("cell0_")
+        if os.path.basename(p).startswith(("alg0_d2", "alg0_d6", "cell0_d2", "cell0_d6"))]
+
+
+@pytest.mark.parametrize("path", LOWD, ids=os.path.basename)
+def test_f32_filter_matches_golden(H, path):
+    """The FP32 filter with its FP64 band recheck gives the fixture's exact
+    neighbour table (2-D and 6-D fixtures, k = 2, 3, 6, cell-0 wrap)."""
+    g = load_golden(path)
+    off, nb, info = gpu_join(H, g["pts"], float(g["eps"]), int(g["k"]), refine="f32")
+    assert info["refine_used"] == "f32"
+    assert np.array_equal(off, g["offsets"]) and np.array_equal(nb, g["nbrs"])
+    assert info["pair_hash"] == int(g["hash"])
+
+
+@pytest.mark.parametrize("dim,qb", [(2, 4), (2, 8), (2, 16), (6, 4), (6, 8)])
+@pytest.mark.parametrize("order", ["morton", "spatial"])
+def test_f32_filter_variants_vs_oracle(H, oracle, dim, qb, order):
+    pts = oracle.generate_expo(30000, dim, seed=40 + dim)
+    eps = 0.002 if dim == 2 else 0.012
+    off, nb, info = gpu_join(H, pts, eps, dim, refine="f32", queries_per_unit=qb, unit_order=order)
+    eoff, enb = oracle.join(pts, dim, eps)
+    assert np.array_equal(off, eoff) and np.array_equal(nb, enb)
+
+
+@pytest.mark.parametrize("dim", [2, 6])
+def test_f32_filter_boundary_pairs_rechecked(H, oracle, dim):
+    """Pairs planted at eps(1 +- 1e-9, 1e-15, 0) fall in the FP32 band and
+    are decided by the FP64 recheck exactly as the oracle does."""
+    eps = 0.01
+    pts = planted_boundary(dim, 2000, eps, seed=60 + dim)
+    off, nb, info = gpu_join(H, pts, eps, dim, refine="f32")
+    eoff, enb = oracle.join(pts, dim, eps)
+    assert info["band_rechecks"] > 0
+    assert np.array_equal(off, eoff) and np.array_equal(nb, enb)
+
+
+def test_f32_filter_declines_wide_bands(H, oracle):
+    """Coordinates far from their centre relative to eps: the FP32 band
+    would exceed 1% of eps^2, so refine="f32" is refused and "auto" runs the
+    FP64 chain."""
+    rng = np.random.default_rng(8)
+    pts = rng.random((3000, 6)) * 1e4
+    pts[::2] = pts[1::2] + 1e-4
+    eps = 1e-3
+    with pytest.raises(H.HegjError) as e:
+        gpu_join(H, pts, eps, 6, refine="f32")
+    assert e.value.status == 2
+    off, nb, info = gpu_join(H, pts, eps, 6, refine="auto")
+    assert info["refine_used"] == "fp64"
+    eoff, enb = oracle.join(pts, 6, eps)
+    assert np.array_equal(off, eoff) and np.array_equal(nb, enb)
+
+
+def test_f32_filter_graph_replay(H, oracle):
+    pts = oracle.generate_expo(20000, 6, seed=78)
+    eoff, enb = oracle.join(pts, 6, 0.012)
+    with H.GridIndex(pts, 0.012, indexed_dims=6) as ix:
+        for _ in range(3):
+            t = ix.selfjoin(use_graph=True, refine="f32")
+            off, nb = t.csr(sort_rows=True)
+            info = t.info
+            t.close()
+            assert info["graph_replay"] == 1 and info["refine_used"] == "f32"
+            assert np.array_equal(off, eoff) and np.array_equal(nb, enb)
+
+
 @pytest.mark.parametrize("n,eps", [(40000, 0.004), (60000, 0.005)])
 def test_large_units_use_global_masks(H, oracle, n, eps):
     """Slabs (k = 1) holding thousands of points: with 16 queries per unit
