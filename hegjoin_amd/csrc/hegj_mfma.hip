@@ -402,14 +402,18 @@ __global__ __launch_bounds__(64 * kGbWaves) void k_gb_join(GbArgs a) {
                                                                               acc[rt][ct], 0, 0, 0);
                 }
             }
-            // reject test: one v_cmp per element; survivors are rare
-            uint64_t anym = 0;
+            // reject test: the lane's smallest s' (min3 folds), one compare
+            // and one wave vote per chunk; survivors are rare.  (s' is never
+            // NaN: padding rows and columns give +2^100 or +inf.)
+            float mn = acc[0][0][0];
 #pragma unroll
             for (int rt = 0; rt < RT; rt++)
 #pragma unroll
                 for (int ct = 0; ct < 4; ct++)
 #pragma unroll
-                    for (int j = 0; j < 4; j++) anym |= __ballot(!(acc[rt][ct][j] > Trej));
+                    for (int j = 0; j < 4; j++)
+                        if (rt | ct | j) mn = fminf(mn, acc[rt][ct][j]);
+            const uint64_t anym = __ballot(!(mn > Trej));
             if (anym) {
 #pragma unroll
                 for (int rt = 0; rt < RT; rt++)
