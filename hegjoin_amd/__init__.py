@@ -85,8 +85,9 @@ class _ServeOpts(C.Structure):
 
 class _RequestInfo(C.Structure):
     _fields_ = [("ticket", C.c_uint64), ("status", C.c_int32), ("level", C.c_uint32),
-                ("n_queries", C.c_uint32), ("reserved", C.c_uint32), ("pairs", C.c_uint64),
+                ("n_queries", C.c_uint32), ("refine_used", C.c_uint32), ("pairs", C.c_uint64),
                 ("pair_hash", C.c_uint64), ("evals", C.c_uint64), ("result_bytes", C.c_uint64),
+                ("band_rechecks", C.c_uint64),
                 ("queue_ms", C.c_double), ("count_ms", C.c_double), ("admit_ms", C.c_double),
                 ("fill_ms", C.c_double), ("latency_ms", C.c_double)]
 

@@ -115,6 +115,7 @@ struct hegj_index {
     hegj::GraphPlan *gplan = nullptr;
     // centred FP32 copy for the FP32 filter (lazily computed)
     float *soa32 = nullptr;
+    double *f32_centre = nullptr;  // device, dim doubles
     double f32_radius = 0;
 };
 
@@ -243,7 +244,9 @@ double f32_bound(double S, double D, double R) {
 // acc > T (twice the bound at S = 1.001 T, which covers every S whose acc or
 // acc32 can reach the thresholds).  False when the band would be wider than
 // 1% of T (then the filter would recheck too much to pay off).
-bool f32_thresholds(double T, uint32_t dim, double R, float *tacc, float *trej) {
+}  // namespace
+
+bool hegj::f32_thresholds(double T, uint32_t dim, double R, float *tacc, float *trej) {
     if (!(T > 1e-30) || !(R < 1e30)) return false;
     const double B = 2.0 * f32_bound(1.001 * T, (double)dim, R);
     if (!(B <= 0.01 * T)) return false;
@@ -256,9 +259,11 @@ bool f32_thresholds(double T, uint32_t dim, double R, float *tacc, float *trej) 
     return true;
 }
 
+namespace {
+
 hegj_status ensure_soa32(hegj_index *h) {
     if (h->soa32) return HEGJ_OK;
-    return make_soa32(h->g, h->ix, &h->soa32, &h->f32_radius, h->stream);
+    return make_soa32(h->g, h->ix, &h->soa32, &h->f32_radius, &h->f32_centre, h->stream);
 }
 
 // Everything a FP64 join needs besides its result: work units in the chosen
@@ -685,6 +690,13 @@ namespace hegj {
 const GridGeom &index_geom(const hegj_index *h) { return h->g; }
 const DevIndex &index_dev(const hegj_index *h) { return h->ix; }
 hegj_status select_device(int device) { return set_device(device); }
+hegj_status index_f32(hegj_index *h, const float **soa32, double *radius, const double **centre) {
+    HEGJ_TRY(ensure_soa32(h));
+    *soa32 = h->soa32;
+    *radius = h->f32_radius;
+    *centre = h->f32_centre;
+    return HEGJ_OK;
+}
 }  // namespace hegj
 
 extern "C" {
@@ -768,6 +780,7 @@ hegj_status hegj_index_free(hegj_index *h) {
     if (h->gb_w) (void)hipFree(h->gb_w);
     if (h->gplan) free_graph_plan(h->gplan);
     if (h->soa32) (void)hipFree(h->soa32);
+    if (h->f32_centre) (void)hipFree(h->f32_centre);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return HEGJ_OK;

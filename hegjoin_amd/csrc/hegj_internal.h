@@ -157,7 +157,11 @@ hegj_status launch_join(const JoinArgs &a, uint32_t qb, uint32_t blocks, hipStre
 // FP32 filter support: centred float copy of the cell-ordered coordinates
 // and the largest |x - centre| (hegj_util.hip).
 hegj_status make_soa32(const GridGeom &g, const DevIndex &ix, float **soa32, double *radius,
-                       hipStream_t s);
+                       double **centre_dev, hipStream_t s);
+// The index's FP32 copy (built on first use), its radius and device centre.
+hegj_status index_f32(hegj_index *h, const float **soa32, double *radius, const double **centre);
+// FP32-filter thresholds for threshold T (false: band wider than 1% of T).
+bool f32_thresholds(double T, uint32_t dim, double R, float *tacc, float *trej);
 // Whether the FP32 filter is instantiated for (dim, qb).
 bool f32_supported(uint32_t dim, uint32_t qb);
 // Zero `bytes` (a multiple of 4) with a kernel (safe inside graph capture).
@@ -210,8 +214,18 @@ struct QJoinArgs {
     uint32_t *out;               // neighbour ids (fill pass)
     unsigned long long *hash;
     unsigned long long *evals;
+    // FP32 filter (f32 != 0): centred float candidates and queries
+    uint32_t f32;
+    const float *soa32;
+    const float *q32;            // [nq * dim] row-major, centred like soa32
+    float tacc32, trej32;
+    unsigned long long *band;    // pairs rechecked in FP64 (fill pass)
 };
 hegj_status query_geom(const GridGeom &g, QueryGeom *qg);
+// q32 = fl32(q - centre); *rmax_bits = bits of max |q - centre| (a positive
+// double, ordered like its bit pattern; must be zeroed by the caller).
+hegj_status query_to_f32(const double *q, uint32_t nq, uint32_t dim, const double *centre, float *q32,
+                         unsigned long long *rmax_bits, hipStream_t s);
 // Accessors of an index handle (hegj_api.cpp).
 const GridGeom &index_geom(const hegj_index *h);
 const DevIndex &index_dev(const hegj_index *h);

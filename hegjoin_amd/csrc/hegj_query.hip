@@ -135,8 +135,12 @@ __device__ uint32_t qbuild_rows(const QJoinArgs &a, uint64_t key, QRowList &rl, 
 
 // D > 0: coordinates in registers; D == 0: any dimensionality, one
 // dimension at a time.
-template <int D, bool FILL>
+// F32: candidates from the level's centred FP32 copy, queries from the
+// request's (centred the same way); acc32 <= tacc32 accepts, acc32 > trej32
+// rejects, the band between is decided by the exact FP64 chain.
+template <int D, bool FILL, bool F32 = false>
 __global__ __launch_bounds__(64 * kQWaves) void k_qjoin(QJoinArgs a) {
+    static_assert(!F32 || D > 0, "FP32 filter needs a fixed dimensionality");
     __shared__ QRowList rls[kQWaves];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     QRowList &rl = rls[wid];
@@ -144,7 +148,7 @@ __global__ __launch_bounds__(64 * kQWaves) void k_qjoin(QJoinArgs a) {
     const uint64_t n = a.qg.g.n;
     const double T = a.thresh;
     const double *__restrict__ soa = a.ix.soa;
-    uint64_t my_hash = 0, my_evals = 0;
+    uint64_t my_hash = 0, my_evals = 0, my_band = 0;
     while (true) {
         uint32_t u = 0;
         if (lane == 0) u = atomicAdd(a.queue_head, 1u);
@@ -169,8 +173,21 @@ __global__ __launch_bounds__(64 * kQWaves) void k_qjoin(QJoinArgs a) {
             // scalar registers), so no load in the candidate loop waits on
             // anything but the candidates
             constexpr int DQ = D > 0 ? D : 1;
-            double qreg[kQB][DQ];
-            if constexpr (D > 0) {
+            double qreg[F32 ? 1 : kQB][DQ];
+            float qregf[F32 ? kQB : 1][DQ];
+            if constexpr (F32) {
+                const int qi = lane / DQ, l = lane % DQ;
+                float v = 0.0f;
+                if (lane < kQB * DQ) {
+                    uint32_t id = 0;
+#pragma unroll
+                    for (int k = 0; k < kQB; k++) id = k == qi ? qid[k] : id;
+                    v = a.q32[(uint64_t)id * dim + l];
+                }
+#pragma unroll
+                for (int j = 0; j < kQB * DQ; j++)
+                    qregf[j / DQ][j % DQ] = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(v), j));
+            } else if constexpr (D > 0) {
                 static_assert(kQB * D <= 64, "one load per lane");
                 const int qi = lane / DQ, l = lane % DQ;
                 double v = 0.0;
@@ -200,7 +217,42 @@ __global__ __launch_bounds__(64 * kQWaves) void k_qjoin(QJoinArgs a) {
                 double acc[kQB];
 #pragma unroll
                 for (int qi = 0; qi < kQB; qi++) acc[qi] = 0.0;
-                if constexpr (D > 0) {
+                if constexpr (F32) {
+                    float x[DQ];
+#pragma unroll
+                    for (int l = 0; l < DQ; l++) x[l] = a.soa32[(uint64_t)l * n + pos];
+                    uint32_t pid = 0;
+                    if (FILL && valid) pid = a.ix.ila[pos];
+#pragma unroll
+                    for (int qi = 0; qi < kQB; qi++) {
+                        float accf = 0.0f;
+#pragma unroll
+                        for (int l = 0; l < DQ; l++) {
+                            const float d = x[l] - qregf[qi][l];
+                            accf = __builtin_fmaf(d, d, accf);
+                        }
+                        const bool live_q = valid && (uint32_t)qi < nqg;
+                        bool hit = live_q && accf <= a.tacc32;
+                        const bool band = live_q && !(accf <= a.tacc32) && !(accf > a.trej32);
+                        if (__ballot(band)) {
+                            double e = 0.0;
+#pragma unroll
+                            for (int l = 0; l < DQ; l++) {
+                                const double d = soa[(uint64_t)l * n + pos] - a.queries[(uint64_t)qid[qi] * dim + l];
+                                e = __builtin_fma(d, d, e);
+                            }
+                            hit |= band && e <= T;
+                            my_band += (uint64_t)__popcll(__ballot(band));
+                        }
+                        const uint64_t m = __ballot(hit);
+                        if (FILL && hit) {
+                            a.out[base[qi] + cnt[qi] + lanes_below(m)] = pid;
+                            my_hash += mix64(((uint64_t)qid[qi] << 32) | pid);
+                        }
+                        cnt[qi] += (uint32_t)__popcll(m);
+                    }
+                    continue;
+                } else if constexpr (D > 0) {
                     double x[DQ];
                     // unconditional (invalid lanes read position 0, masked below)
 #pragma unroll
@@ -252,7 +304,24 @@ __global__ __launch_bounds__(64 * kQWaves) void k_qjoin(QJoinArgs a) {
     if (lane == 0) {
         if (FILL && my_hash) atomicAdd(a.hash, (unsigned long long)my_hash);
         if (!FILL && my_evals) atomicAdd(a.evals, (unsigned long long)my_evals);
+        if (FILL && my_band && a.band) atomicAdd(a.band, (unsigned long long)my_band);
     }
+}
+
+// q32 = fl32(fl64(q - centre)) and the largest |q - centre| (as ordered bits).
+__global__ __launch_bounds__(256) void k_q32(const double *__restrict__ q, uint64_t total, uint32_t dim,
+                                             const double *__restrict__ centre, float *__restrict__ q32,
+                                             unsigned long long *rmax_bits) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    double r = 0.0;
+    if (i < total) {
+        const double d = q[i] - centre[i % dim];
+        q32[i] = (float)d;
+        r = fabs(d);
+    }
+    for (int o = 32; o > 0; o >>= 1) r = fmax(r, __shfl_xor(r, o, 64));
+    if ((threadIdx.x & 63) == 0 && r > 0.0)
+        atomicMax(rmax_bits, (unsigned long long)__double_as_longlong(r));
 }
 
 __global__ __launch_bounds__(256) void k_q_u32_u64(const uint32_t *__restrict__ in, uint32_t n,
@@ -263,7 +332,15 @@ __global__ __launch_bounds__(256) void k_q_u32_u64(const uint32_t *__restrict__ 
 
 template <int D, bool FILL>
 hegj_status launch_q(const QJoinArgs &a, uint32_t blocks, hipStream_t s) {
-    k_qjoin<D, FILL><<<blocks, 64 * kQWaves, 0, s>>>(a);
+    if constexpr (D > 0) {
+        if (a.f32)
+            k_qjoin<D, FILL, true><<<blocks, 64 * kQWaves, 0, s>>>(a);
+        else
+            k_qjoin<D, FILL><<<blocks, 64 * kQWaves, 0, s>>>(a);
+    } else {
+        if (a.f32) return fail(HEGJ_ERR_UNSUPPORTED, "FP32 filter needs dim 2 or 6");
+        k_qjoin<D, FILL><<<blocks, 64 * kQWaves, 0, s>>>(a);
+    }
     HIP_TRY(hipGetLastError());
     return HEGJ_OK;
 }
@@ -346,6 +423,15 @@ hegj_status query_run(const QJoinArgs &base, bool fill, int device, hipStream_t 
     const uint32_t blocks =
         std::max<uint32_t>(1, std::min<uint32_t>(resident, ceil_div(base.n_units, kQWaves)));
     return fill ? launch_qd<true>(base, blocks, s) : launch_qd<false>(base, blocks, s);
+}
+
+hegj_status query_to_f32(const double *q, uint32_t nq, uint32_t dim, const double *centre, float *q32,
+                         unsigned long long *rmax_bits, hipStream_t s) {
+    const uint64_t total = (uint64_t)nq * dim;
+    if (total == 0) return HEGJ_OK;
+    k_q32<<<ceil_div(total, 256), 256, 0, s>>>(q, total, dim, centre, q32, rmax_bits);
+    HIP_TRY(hipGetLastError());
+    return HEGJ_OK;
 }
 
 hegj_status query_offsets(const uint32_t *counts, uint32_t nq, uint64_t *offsets, hipStream_t s) {

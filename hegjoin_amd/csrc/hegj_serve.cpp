@@ -93,7 +93,8 @@ struct Request {
     std::string error;
     bool holds_pool = false;
     uint64_t pool_off = 0, pool_bytes = 0;
-    uint64_t pairs = 0, hash = 0, evals = 0;
+    uint64_t pairs = 0, hash = 0, evals = 0, band = 0;
+    bool f32 = false;
     Clock::time_point t_submit, t_start, t_counted, t_admit, t_done;
 };
 
@@ -107,6 +108,12 @@ struct hegj_server {
     int device = 0;
     std::vector<hegj_index *> levels;
     std::vector<QueryGeom> qgeo;
+    struct LevelF32 {
+        const float *soa32 = nullptr;
+        const double *centre = nullptr;
+        double radius = 0;
+    };
+    std::vector<LevelF32> lf32;  // FP32 copies (dim 2 and 6)
     char *pool_base = nullptr;
     Pool pool;
     hipStream_t copy_stream = nullptr;
@@ -170,10 +177,27 @@ hegj_status hegj_server::run(Request *r, hipStream_t s, Arena &arena, void *ctr_
     HEGJ_TRY(query_plan(&plan, keep, qg, q, nq, s));
     struct Ctr {
         uint32_t head[2];
-        unsigned long long hash, evals;
+        unsigned long long hash, evals, band, rbits;
     };
     Ctr *ctr = static_cast<Ctr *>(ctr_mem);
     HIP_TRY(hipMemsetAsync(ctr, 0, sizeof(Ctr), s));
+    // FP32 filter: queries centred like the level's copy; the band comes from
+    // the larger of the level's and the queries' radius around the centre
+    bool f32 = false;
+    float tacc = 0, trej = 0;
+    float *q32 = nullptr;
+    if (!lf32.empty() && lf32[r->level].soa32) {
+        const LevelF32 &L = lf32[r->level];
+        HIP_TRY(keep.get(&q32, (size_t)nq * dim));
+        HEGJ_TRY(query_to_f32(q, nq, dim, L.centre, q32, &ctr->rbits, s));
+        unsigned long long rb = 0;
+        HIP_TRY(hipMemcpyAsync(&rb, &ctr->rbits, sizeof(rb), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        double rq;
+        std::memcpy(&rq, &rb, sizeof(rq));
+        const double R = std::max(L.radius, rq) * (1.0 + 1e-12);
+        f32 = f32_thresholds(sqrt_threshold(r->eps), dim, R, &tacc, &trej);
+    }
     HIP_TRY(hipMemsetAsync(counts, 0, sizeof(uint32_t) * nq, s));
     QJoinArgs a{};
     a.qg = qg;
@@ -188,6 +212,14 @@ hegj_status hegj_server::run(Request *r, hipStream_t s, Arena &arena, void *ctr_
     a.counts = counts;
     a.hash = &ctr->hash;
     a.evals = &ctr->evals;
+    a.band = &ctr->band;
+    if (f32) {
+        a.f32 = 1;
+        a.soa32 = lf32[r->level].soa32;
+        a.q32 = q32;
+        a.tacc32 = tacc;
+        a.trej32 = trej;
+    }
     HEGJ_TRY(query_run(a, false, device, s));
     HEGJ_TRY(query_offsets(counts, nq, offsets, s));
     uint64_t total = 0;
@@ -210,6 +242,8 @@ hegj_status hegj_server::run(Request *r, hipStream_t s, Arena &arena, void *ctr_
     r->pairs = total;
     r->hash = c.hash;
     r->evals = c.evals;
+    r->band = c.band;
+    r->f32 = f32;
     return HEGJ_OK;
 }
 
@@ -335,6 +369,15 @@ hegj_status hegj_serve_open(const double *database, uint32_t n, const hegj_serve
             return bail(st);
         }
         S->levels.push_back(h);
+        if (o->dim == 2 || o->dim == 6) {
+            hegj_server::LevelF32 L;
+            st = index_f32(h, &L.soa32, &L.radius, &L.centre);
+            if (st != HEGJ_OK) {
+                (void)hipFree(dev);
+                return bail(st);
+            }
+            S->lf32.push_back(L);
+        }
         QueryGeom qg;
         st = query_geom(index_geom(h), &qg);
         if (st != HEGJ_OK) {
@@ -410,6 +453,8 @@ static hegj_status finished_info(const Request *r, hegj_request_info *info) {
         info->pairs = r->pairs;
         info->pair_hash = r->hash;
         info->evals = r->evals;
+        info->band_rechecks = r->band;
+        info->refine_used = r->f32 ? HEGJ_REFINE_F32 : HEGJ_REFINE_FP64;
         info->result_bytes = r->holds_pool ? r->pool_bytes : 0;
         info->queue_ms = ms_between(r->t_submit, r->t_start);
         if (r->state == kDone) {

@@ -374,13 +374,14 @@ __global__ __launch_bounds__(256) void k_soa32(const double *__restrict__ soa, u
 }  // namespace
 
 hegj_status make_soa32(const GridGeom &g, const DevIndex &ix, float **soa32, double *radius,
-                       hipStream_t s) {
+                       double **centre_dev, hipStream_t s) {
     const uint32_t n = g.n, dim = g.dim;
     const uint32_t blocks = std::min<uint32_t>(ceil_div(n, 256), 512);
     Scratch sc;
     double *part, *centre;
     HIP_TRY(sc.get(&part, (size_t)blocks * dim * 2));
-    HIP_TRY(sc.get(&centre, dim));
+    HIP_TRY(hipMalloc(centre_dev, sizeof(double) * dim));
+    centre = *centre_dev;
     k_dim_minmax<<<blocks, 256, 0, s>>>(ix.soa, n, dim, part);
     HIP_TRY(hipGetLastError());
     std::vector<double> hp((size_t)blocks * dim * 2), c(dim);

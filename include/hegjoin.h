@@ -277,11 +277,12 @@ typedef struct {
     int32_t status;            /* hegj_status of the request */
     uint32_t level;            /* grid level used */
     uint32_t n_queries;
-    uint32_t reserved;
+    uint32_t refine_used;      /* HEGJ_REFINE_F32 (dim 2, 6, narrow band) or HEGJ_REFINE_FP64 */
     uint64_t pairs;            /* result size */
     uint64_t pair_hash;        /* sum of mix64(query<<32 | point) over pairs */
     uint64_t evals;            /* distance evaluations */
     uint64_t result_bytes;     /* pool bytes held by the result */
+    uint64_t band_rechecks;    /* pairs decided by the FP64 recheck (FP32 filter) */
     double queue_ms;           /* submit -> on a slot */
     double count_ms;           /* on a slot -> result size known */
     double admit_ms;           /* waiting for pool memory */

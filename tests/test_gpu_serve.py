@@ -54,6 +54,7 @@ def test_requests_match_bruteforce(H, oracle, base):
             assert info["pairs"] == len(enb)
             assert info["pair_hash"] == oracle.csr_hash(eoff, enb)
             assert SIDES[info["level"]] >= eps
+            assert info["refine_used"] == "f32"  # 6-D, data in [0, 1]: narrow band
 
 
 def test_alone_and_in_a_mix_identical(H, oracle, base):
@@ -148,6 +149,25 @@ def test_concurrent_clients(H, oracle, base):
     assert st["completed"] == len(work) and st["p99_ms"] >= st["p50_ms"] > 0
     for (off, nb), (eoff, enb) in zip(out, expect):
         assert np.array_equal(off, eoff) and np.array_equal(sorted_rows(off, nb), enb)
+
+
+def test_boundary_pairs_decided_by_fp64_recheck(H, oracle):
+    """Queries planted at eps(1 +- 1e-9, 1e-15, 0) from index points: they
+    land in the FP32 band and the FP64 recheck decides them as the oracle."""
+    rng = np.random.default_rng(12)
+    base = 0.3 + 0.4 * rng.random((3000, 6))
+    d = rng.standard_normal((3000, 6))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    eps = 0.015
+    rel = rng.choice([-1e-9, -1e-15, 0.0, 1e-15, 1e-9], 3000)
+    q = base + d * (eps * (1 + rel))[:, None]
+    with H.Server(base, SIDES, slots=2, result_bytes=64 << 20, max_queries=5000) as S:
+        t = S.submit(q, eps)
+        info = S.wait(t)
+        off, nb = S.result(t)
+    eoff, enb = oracle.range_join(base, q, eps)
+    assert info["refine_used"] == "f32" and info["band_rechecks"] > 0
+    assert np.array_equal(off, eoff) and np.array_equal(sorted_rows(off, nb), enb)
 
 
 def test_queries_outside_the_grid_and_empty_requests(H, oracle, base):
