@@ -72,11 +72,13 @@ def cpu_baseline(pts, eps, dim, seconds_budget=20.0):
     from oracle import pyoracle as O
     n = len(pts)
     order = np.random.RandomState(7).permutation(n).astype(np.uint32)
-    # probe 2000 queries to size a sample of about seconds_budget of join time
+    # probe 20000 queries (small probes overstate the per-query cost: thread
+    # start-up) to size a sample of about seconds_budget of join time
     if O.ref_available(dim):
-        probe = O.ref_superego_time(pts, eps, order, 2000)
-        per_q = max(probe["join_s"], 1e-6) / 2000
-        nq = int(min(n, max(2000, seconds_budget / per_q)))
+        probe_n = min(n, 20000)
+        probe = O.ref_superego_time(pts, eps, order, probe_n)
+        per_q = max(probe["join_s"], 1e-6) / probe_n
+        nq = int(min(n, max(probe_n, seconds_budget / per_q)))
         r = O.ref_superego_time(pts, eps, order, nq)
         return {"value": r["pairs"] / r["join_s"], "unit": "result pairs/s",
                 "cores": r["threads"], "kind": "reference",
