@@ -130,17 +130,27 @@ __global__ __launch_bounds__(kT) void k_gather_soa(const double *__restrict__ pt
 
 }  // namespace
 
-// Host side of S1 (main.cu:493-518) from the exact device min/max.
-void finish_dims(const double *mn, const double *mx, uint32_t k, double eps, GridGeom *g,
+// Host side of S1 (main.cu:493-518) from the exact device min/max.  Returns
+// false when the grid cannot be represented: a dimension with 2^32 cells or
+// more, or more than 2^64 - 1 cells in total (the reference's uint32 cast and
+// uint64 product would wrap there and cell ids would collide).
+bool finish_dims(const double *mn, const double *mx, uint32_t k, double eps, GridGeom *g,
                  double *maxArr, uint64_t *totalCells) {
     for (uint32_t j = 0; j < k; j++) {
         g->minArr[j] = mn[j] - eps;
         maxArr[j] = mx[j] + eps;
     }
-    for (uint32_t j = 0; j < k; j++) g->nCells[j] = (uint32_t)std::ceil((maxArr[j] - g->minArr[j]) / eps);
-    uint64_t t = g->nCells[0];
-    for (uint32_t j = 1; j < k; j++) t *= g->nCells[j];
-    *totalCells = t;
+    unsigned __int128 t = 1;
+    bool ok = true;
+    for (uint32_t j = 0; j < k; j++) {
+        const double c = std::ceil((maxArr[j] - g->minArr[j]) / eps);
+        ok &= c < 4294967296.0;
+        g->nCells[j] = ok ? (uint32_t)c : 1u;
+        t *= g->nCells[j];
+        ok &= t <= (unsigned __int128)~0ull;
+    }
+    *totalCells = ok ? (uint64_t)t : 0;
+    return ok;
 }
 
 // Builds the device index from device points (already copied into ix->pts).
@@ -166,7 +176,10 @@ hegj_status build_index(GridGeom *g, double *maxArr, uint64_t *totalCells, DevIn
             mx[j] = std::max(mx[j], hp[(size_t)b * 2 * k + 2 * j + 1]);
         }
     }
-    finish_dims(mn, mx, k, g->eps, g, maxArr, totalCells);
+    if (!finish_dims(mn, mx, k, g->eps, g, maxArr, totalCells))
+        return fail(HEGJ_ERR_UNSUPPORTED,
+                    "grid too fine: more than 2^64 cells (or 2^32 in one dimension); index fewer "
+                    "dimensions or use a larger epsilon");
     // key bits
     int bits = 0;
     while (bits < 64 && ((*totalCells - 1) >> bits) != 0) bits++;

@@ -276,16 +276,26 @@ def test_f32_filter_declines_wide_bands(H, oracle):
     would exceed 1% of eps^2, so refine="f32" is refused and "auto" runs the
     FP64 chain."""
     rng = np.random.default_rng(8)
-    pts = rng.random((3000, 6)) * 1e4
+    pts = rng.random((3000, 6)) * 1e3
     pts[::2] = pts[1::2] + 1e-4
     eps = 1e-3
     with pytest.raises(H.HegjError) as e:
-        gpu_join(H, pts, eps, 6, refine="f32")
+        gpu_join(H, pts, eps, 2, refine="f32")
     assert e.value.status == 2
-    off, nb, info = gpu_join(H, pts, eps, 6, refine="auto")
+    off, nb, info = gpu_join(H, pts, eps, 2, refine="auto")
     assert info["refine_used"] == "fp64"
-    eoff, enb = oracle.join(pts, 6, eps)
+    eoff, enb = oracle.join(pts, 2, eps)
+    assert len(enb) > 3000
     assert np.array_equal(off, eoff) and np.array_equal(nb, enb)
+
+
+def test_grid_with_too_many_cells_is_refused(H):
+    """1e6 cells per dimension in 6 dimensions: the linear cell id would need
+    more than 64 bits (the reference's product wraps), so the build fails."""
+    pts = np.random.default_rng(2).random((1000, 6)) * 1e3
+    with pytest.raises(H.HegjError) as e:
+        H.GridIndex(pts, 1e-3, indexed_dims=6)
+    assert e.value.status == 2
 
 
 def test_f32_filter_graph_replay(H, oracle):
