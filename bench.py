@@ -101,7 +101,9 @@ def rooflines(n, dim, cells, pairs, evals, kernel_ms, refine):
       HBM bytes   = 8 n D (coordinates once) + 4 n (indexLookupArr)
                     + 24 C (grid + gridCellLookup per cell) + 8 P (query and
                     neighbour id per pair)
-      FP64 flops  = 3 D per reference candidate E (subtract + fma per dim)
+      FP64 flops  = 3 D per distance evaluation E the kernel performs
+                    (subtract + fma per dim; E is the reference's candidate
+                    count less the candidates cell-box pruning leaves out)
       MFMA flops  = 2 D per candidate pair, against the bf16 peak divided by
                     the 3 products of a split-BF16 dot (hi.hi + hi.lo + lo.hi)
     Returns (binding, other): the binding roof is the one whose roof time is
@@ -120,12 +122,12 @@ def rooflines(n, dim, cells, pairs, evals, kernel_ms, refine):
     elif refine == "f32":
         flops = 3.0 * dim * evals
         peak = PEAK_FP32_TFLOPS
-        note = ("3 D FP32 flops (sub + fma per dim) per reference candidate, FP32 vector peak; "
+        note = ("3 D FP32 flops (sub + fma per dim) per distance evaluation, FP32 vector peak; "
                 "kernel k_join with the FP32 filter")
     else:
         flops = 3.0 * dim * evals
         peak = PEAK_FP64_TFLOPS
-        note = "3 D FP64 flops (sub + fma per dim) per reference candidate; kernel k_join"
+        note = "3 D FP64 flops (sub + fma per dim) per distance evaluation; kernel k_join"
     comp = {"bound": "mfma", "achieved": flops / t / 1e12, "peak": peak, "unit": "TFLOP/s",
             "frac": flops / t / 1e12 / peak, "traffic": None, "work": flops,
             "roof_ms": flops / (peak * 1e12) * 1e3, "note": note}
@@ -273,6 +275,8 @@ def main():
     ap.add_argument("--graph", action="store_true",
                     help="replay the captured batch graph (FP64 refine)")
     ap.add_argument("--n", type=int, default=0, help="override the point count (experiments)")
+    ap.add_argument("--no-prune", action="store_true",
+                    help="evaluate every point of every stencil cell (no cell-box pruning)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -304,7 +308,7 @@ def main():
     print(f"# index: {iinfo['nNonEmptyCells']} cells, build {iinfo['build_ms']:.1f} ms, "
           f"candidates {total_cand}", file=sys.stderr, flush=True)
     # options not given on the command line take the tuned defaults (tuning.tsv)
-    kw = dict(part_by_block=a.parts_by_block, use_graph=a.graph)
+    kw = dict(part_by_block=a.parts_by_block, use_graph=a.graph, prune=not a.no_prune)
     for key, val in (("queries_per_unit", a.queries_per_unit), ("unit_order", a.unit_order),
                      ("xcd_parts", a.xcd_parts), ("units_per_grab", a.grab), ("refine", a.refine)):
         if val is not None:
@@ -329,7 +333,10 @@ def main():
     kernel_ms = sum(i["join_ms"] for i in infos) / launches
     refine_used = infos[-1]["refine_used"]
     per_launch = launches / a.steps
-    roof, roof_alt = rooflines(n, dim, iinfo["nNonEmptyCells"], pairs, total_cand / per_launch,
+    # compute work: the distance evaluations the kernel performed (cell-box
+    # pruning leaves out candidates the reference evaluates)
+    evals_launch = sum(i["evals_executed"] for i in infos) / launches
+    roof, roof_alt = rooflines(n, dim, iinfo["nNonEmptyCells"], pairs, evals_launch,
                                kernel_ms, refine_used)
     roof_alt["attainable_copy_gbs"] = copy_gbs
     attach_traffic(roof, a.config, refine_used, kernel_ms)
@@ -346,7 +353,9 @@ def main():
                    "eps": eps, "pairs": pairs, "total_candidates": total_cand,
                    "cells": iinfo["nNonEmptyCells"], "cell0_silenced": iinfo["n_cell0_queries"],
                    "parallelism": "1 GPU"},
-        "distance_evals_per_s": total_cand * a.steps / elapsed,
+        "distance_evals_per_s": sum(i["evals_executed"] for i in infos) / elapsed,
+        "evals_vs_reference_candidates": infos[-1]["evals_executed"] / max(total_cand, 1),
+        "prune": not a.no_prune,
         "pair_hash": "%016x" % infos[0]["pair_hash"],
         "join_kernel_ms": kernel_ms,
         "estimate_ms": infos[-1]["estimate_ms"],

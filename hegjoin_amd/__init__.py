@@ -63,7 +63,8 @@ class _JoinOpts(C.Structure):
     _fields_ = [("result_capacity", C.c_uint64), ("unit_order", C.c_uint32),
                 ("queries_per_unit", C.c_uint32), ("refine", C.c_uint32),
                 ("xcd_parts", C.c_uint32), ("units_per_grab", C.c_uint32),
-                ("part_by_block", C.c_uint32), ("use_graph", C.c_uint32)]
+                ("part_by_block", C.c_uint32), ("use_graph", C.c_uint32),
+                ("no_prune", C.c_uint32)]
 
 
 class _ResultInfo(C.Structure):
@@ -304,7 +305,7 @@ class GridIndex:
         return cells, opi, tot.value
 
     def selfjoin(self, result_capacity=0, unit_order=None, queries_per_unit=None, xcd_parts=None,
-                 refine=None, units_per_grab=None, part_by_block=0, use_graph=False):
+                 refine=None, units_per_grab=None, part_by_block=0, use_graph=False, prune=True):
         """Options left at None take the tuned defaults for this
         dimensionality (tuning.tsv, written by hegjoin_amd.autotune).
         unit_order: "morton", "spatial" (linear cell id) or "workload"
@@ -313,7 +314,9 @@ class GridIndex:
         recheck) or "f32" (FP32 distance filter + exact FP64 recheck of its
         band); xcd_parts: unit ranges, one per XCD (0 = auto);
         use_graph: keep the FP64 plan in the index and replay a captured
-        hipGraph of the batch (repeated joins with the same options)."""
+        hipGraph of the batch (repeated joins with the same options);
+        prune=False evaluates every point of every stencil cell (the pair set
+        is the same either way)."""
         o = _JoinOpts()
         _lib.hegj_join_opts_tuned(C.byref(o), self.dim)
         o.result_capacity = result_capacity
@@ -329,6 +332,7 @@ class GridIndex:
             o.units_per_grab = units_per_grab
         o.part_by_block = part_by_block
         o.use_graph = 1 if use_graph else 0
+        o.no_prune = 0 if prune else 1
         r = _vp()
         _check(_lib.hegj_selfjoin(self._h, C.byref(o), C.byref(r)))
         return NeighborTable(r, self)

@@ -115,6 +115,7 @@ struct hegj_index {
     hegj::GraphPlan *gplan = nullptr;
     // centred FP32 copy for the FP32 filter (lazily computed)
     float *soa32 = nullptr;
+    float *aos32 = nullptr;        // point-major, dim <= kAosMaxDim
     double *f32_centre = nullptr;  // device, dim doubles
     double f32_radius = 0;
 };
@@ -147,6 +148,7 @@ hegj_status set_device(int dev) {
 hegj_status check_opts(const hegj_index_opts *o, uint32_t n, double eps) {
     if (!o) return fail(HEGJ_ERR_INVALID_ARG, "opts is NULL");
     if (n == 0) return fail(HEGJ_ERR_INVALID_ARG, "empty dataset");
+    if (n > HEGJ_MAX_POINTS) return fail(HEGJ_ERR_UNSUPPORTED, "more than HEGJ_MAX_POINTS (2^29 - 1) points");
     if (o->dim == 0) return fail(HEGJ_ERR_INVALID_ARG, "dim must be >= 1");
     if (o->indexed_dims == 0 || o->indexed_dims > HEGJ_MAX_INDEXED_DIMS || o->indexed_dims > o->dim)
         return fail(HEGJ_ERR_INVALID_ARG, "indexed_dims must be in [1, min(dim, 6)]");
@@ -263,7 +265,7 @@ namespace {
 
 hegj_status ensure_soa32(hegj_index *h) {
     if (h->soa32) return HEGJ_OK;
-    return make_soa32(h->g, h->ix, &h->soa32, &h->f32_radius, &h->f32_centre, h->stream);
+    return make_soa32(h->g, h->ix, &h->soa32, &h->aos32, &h->f32_radius, &h->f32_centre, h->stream);
 }
 
 // Everything a FP64 join needs besides its result: work units in the chosen
@@ -328,10 +330,12 @@ JoinArgs base_args(const hegj_index *h, const hegj_join_opts &opts, const JoinPl
     a.gmask = P.gmask;
     a.gmask_slots = P.gmask_slots;
     a.part_by_xcc = opts.part_by_block ? 0u : 1u;
+    a.prune = opts.no_prune ? 0u : 1u;
     a.band = &P.d_ctr->band;
     if (P.f32) {
         a.f32 = 1;
         a.soa32 = h->soa32;
+        a.aos32 = h->aos32;
         a.tacc32 = P.tacc32;
         a.trej32 = P.trej32;
     }
@@ -512,7 +516,7 @@ hegj_status run_batches(hegj_index *h, const hegj_join_opts &opts, uint32_t qb, 
 }  // namespace
 
 struct hegj::GraphPlan {
-    uint32_t key[8] = {};
+    uint32_t key[9] = {};
     JoinPlan P;
     uint32_t qb = 0, blocks = 0, grab = 1;
     JoinArgs *d_args = nullptr, *h_args = nullptr;
@@ -561,9 +565,9 @@ hegj_status selfjoin_graph(hegj_index *h, const hegj_join_opts &opts, uint32_t q
                            const uint32_t *order, hegj_result *r, bool *fits) {
     hipStream_t s = h->stream;
     const uint32_t n = h->g.n;
-    const uint32_t key[8] = {qb, opts.unit_order, opts.xcd_parts, opts.units_per_grab,
+    const uint32_t key[9] = {qb, opts.unit_order, opts.xcd_parts, opts.units_per_grab,
                              opts.part_by_block, (uint32_t)opts.result_capacity,
-                             (uint32_t)(opts.result_capacity >> 32), opts.refine};
+                             (uint32_t)(opts.result_capacity >> 32), opts.refine, opts.no_prune};
     if (h->gplan && std::memcmp(h->gplan->key, key, sizeof(key)) != 0) {
         free_graph_plan(h->gplan);
         h->gplan = nullptr;
@@ -780,6 +784,7 @@ hegj_status hegj_index_free(hegj_index *h) {
     if (h->gb_w) (void)hipFree(h->gb_w);
     if (h->gplan) free_graph_plan(h->gplan);
     if (h->soa32) (void)hipFree(h->soa32);
+    if (h->aos32) (void)hipFree(h->aos32);
     if (h->f32_centre) (void)hipFree(h->f32_centre);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;

@@ -128,6 +128,16 @@ __global__ __launch_bounds__(kT) void k_gather_soa(const double *__restrict__ pt
     for (uint32_t l = 0; l < dim; l++) soa[(uint64_t)l * n + p] = src[l];
 }
 
+// aos[p*dim + l] = pts[ila[p]*dim + l]: the same, point-major.
+__global__ __launch_bounds__(kT) void k_gather_aos(const double *__restrict__ pts,
+                                                   const uint32_t *__restrict__ ila, uint32_t n,
+                                                   uint32_t dim, double *__restrict__ aos) {
+    uint32_t p = blockIdx.x * kT + threadIdx.x;
+    if (p >= n) return;
+    const double *src = pts + (uint64_t)ila[p] * dim;
+    for (uint32_t l = 0; l < dim; l++) aos[(uint64_t)p * dim + l] = src[l];
+}
+
 }  // namespace
 
 // Host side of S1 (main.cu:493-518) from the exact device min/max.  Returns
@@ -246,12 +256,17 @@ hegj_status build_index(GridGeom *g, double *maxArr, uint64_t *totalCells, DevIn
     HIP_TRY(hipMalloc(&ix->soa, sizeof(double) * (size_t)n * dim));
     k_gather_soa<<<blocks, kT, 0, s>>>(ix->pts, ix->ila, n, dim, ix->soa);
     HIP_TRY(hipGetLastError());
+    if (dim <= kAosMaxDim) {
+        HIP_TRY(hipMalloc(&ix->aos, sizeof(double) * (size_t)n * dim));
+        k_gather_aos<<<blocks, kT, 0, s>>>(ix->pts, ix->ila, n, dim, ix->aos);
+        HIP_TRY(hipGetLastError());
+    }
     HIP_TRY(hipStreamSynchronize(s));
     return HEGJ_OK;
 }
 
 void free_index(DevIndex *ix) {
-    void *ps[] = {ix->pts, ix->soa, ix->ila, ix->point_key, ix->cell_key, ix->cell_start,
+    void *ps[] = {ix->pts, ix->soa, ix->aos, ix->ila, ix->point_key, ix->cell_key, ix->cell_start,
                   ix->row_start, ix->row_key, ix->hkeys, ix->hvals};
     for (void *p : ps)
         if (p) (void)hipFree(p);

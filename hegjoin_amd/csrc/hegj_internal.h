@@ -38,6 +38,7 @@ constexpr int kWave = 64;          // CDNA wavefront width
 constexpr int kMaxK = HEGJ_MAX_INDEXED_DIMS;
 constexpr int kMaxRows = 243;      // 3^(kMaxK-1) stencil rows
 constexpr int kMaxBatches = 64;    // result batches per join
+constexpr uint32_t kAosMaxDim = 8; // point-major copies of the coordinates up to this dim
 
 // Grid geometry passed by value to kernels.
 struct GridGeom {
@@ -58,6 +59,8 @@ struct GridGeom {
 struct DevIndex {
     double *pts = nullptr;        // [n*dim] AoS input copy (dev_database)
     double *soa = nullptr;        // [dim][n] coordinates in cell order
+    double *aos = nullptr;        // [n][dim] the same, point-major (dim <= kAosMaxDim): the
+                                  // join's candidate loads touch one run of bytes per slot
     uint32_t *ila = nullptr;      // [n] indexLookupArr: cell-order position -> point id
     uint64_t *point_key = nullptr;// [n] linear cell id per point (original order)
     uint64_t *cell_key = nullptr; // [C] gridLinearID of each non-empty cell
@@ -133,10 +136,12 @@ struct JoinArgs {
     unsigned long long *evals;    // distance evaluations performed
     unsigned long long *pairs;    // pairs found (count-only) or written
     uint32_t count_only;
+    uint32_t prune;               // 1: cell-box pruning of stencil cells
     // FP32 filter (f32 != 0): centred float coordinates and the two
     // thresholds that make acc32 decide exactly outside the band
     uint32_t f32;
     const float *soa32;           // [dim][n], cell order
+    const float *aos32;           // [n][dim], cell order (dim <= kAosMaxDim)
     float tacc32, trej32;
     unsigned long long *band;     // pairs rechecked in FP64
 };
@@ -155,8 +160,9 @@ hegj_status build_units(const GridGeom &g, const DevIndex &ix, uint32_t qb, cons
 hegj_status launch_join(const JoinArgs &a, uint32_t qb, uint32_t blocks, hipStream_t s,
                         const JoinArgs *dev_args = nullptr);
 // FP32 filter support: centred float copy of the cell-ordered coordinates
-// and the largest |x - centre| (hegj_util.hip).
-hegj_status make_soa32(const GridGeom &g, const DevIndex &ix, float **soa32, double *radius,
+// (SoA, and point-major into *aos32 when aos32 != nullptr and dim <=
+// kAosMaxDim) and the largest |x - centre| (hegj_util.hip).
+hegj_status make_soa32(const GridGeom &g, const DevIndex &ix, float **soa32, float **aos32, double *radius,
                        double **centre_dev, hipStream_t s);
 // The index's FP32 copy (built on first use), its radius and device centre.
 hegj_status index_f32(hegj_index *h, const float **soa32, double *radius, const double **centre);

@@ -42,15 +42,30 @@ constexpr int kWavesPerBlockQ = kWavesPerBlock;
 constexpr uint32_t kMaxQueryLds = 36u << 10;
 
 // Compacted stencil-row list of one unit in LDS.
+constexpr int kBandList = 64;  // FP32 filter: (query, chunk) pairs awaiting the FP64 recheck
 struct RowList {
     uint32_t start[kMaxRows];    // first candidate position of slot s
     uint32_t off[kMaxRows + 1];  // prefix of slot lengths
+    float2 gap[16 * kMaxK];      // per (query, indexed dim): squared gaps to the -1 / +1 cell
+    uint32_t band[kBandList];    // (chunk << 4) | query
 };
 
 // Enumerate the stencil rows of cell `cell` into rl; returns the number of
 // candidates M and the number of slots through *nslots.
+//
+// Cell-box pruning: a stencil cell whose box lies farther than eps from every
+// query of the unit cannot hold a neighbour, so it is left out.  With f_j the
+// query's position inside its cell along indexed dim j (in units of eps), the
+// gap to the neighbour cell at offset -1 is f_j, at +1 it is 1 - f_j; a cell
+// is kept when min over the unit's queries of sum_j gap_j^2 <= T / eps^2.
+// Each gap is shrunk by 1e-6 + 2e-15 (c_j + 2) (the FP64 rounding of the
+// cell assignment (x - min) / eps at coordinate c_j) and the bound raised by
+// 1e-5 relative (FP32 squares and sums), so a kept set can only be larger
+// than the exact one: the pair set is unchanged, fewer candidates are
+// evaluated.  All points of a unit lie in one cell.
+template <int QB>
 __device__ uint32_t build_rows(const JoinArgs &a, uint32_t cell, RowList &rl, uint32_t *nslots,
-                               bool *live) {
+                               bool *live, uint32_t qpos0, uint32_t nq) {
     const GridGeom &g = a.g;
     const int lane = threadIdx.x & 63;
     uint32_t c[kMaxK], lo[kMaxK], hi[kMaxK];
@@ -62,14 +77,65 @@ __device__ uint32_t build_rows(const JoinArgs &a, uint32_t cell, RowList &rl, ui
         *nslots = 0;
         return 0;
     }
-    const uint32_t nrows = pow3(g.k - 1);
+    // squared gaps of query q to its -1 / +1 neighbour along indexed dim j,
+    // into rl.gap[q * kMaxK + j] (read back below as wave-uniform broadcasts)
+    static_assert(QB <= 16, "gap table holds 16 queries");
+    const uint32_t k = g.k;
+    for (uint32_t e = (uint32_t)lane; e < nq * k; e += 64) {
+        const uint32_t qi = e / k, j = e - qi * k;
+        uint32_t cj = 0;
+#pragma unroll
+        for (int jj = 0; jj < kMaxK; jj++)
+            if ((uint32_t)jj == j) cj = c[jj];
+        const double x = a.ix.soa[(uint64_t)j * g.n + qpos0 + qi];
+        double f = (x - g.minArr[j]) / g.eps - (double)cj;
+        f = f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
+        const double mg = 1e-6 + 2e-15 * ((double)cj + 2.0);
+        const double gm = f - mg > 0.0 ? f - mg : 0.0, gp = 1.0 - f - mg > 0.0 ? 1.0 - f - mg : 0.0;
+        rl.gap[qi * kMaxK + j] = make_float2((float)(gm * gm), (float)(gp * gp));
+    }
+    wave_lds_sync();
+    const float thr = a.prune ? (float)(g.thresh / (g.eps * g.eps)) * (1.0f + 1e-5f) : INFINITY;
+    const uint32_t nrows = pow3(k - 1);
     uint32_t slots = 0, total = 0;
     for (uint32_t r0 = 0; r0 < nrows; r0 += 64) {
-        uint32_t r = r0 + lane;
+        const uint32_t r = r0 + lane;
         uint32_t pstart = 0, plen = 0;
-        if (r < nrows) {
+        // offset digit of this row along indexed dims 1..k-1 (0: -1, 1: 0, 2: +1)
+        uint32_t o[kMaxK];
+        {
+            uint32_t rr = r;
+#pragma unroll
+            for (int j = 1; j < kMaxK; j++) {
+                o[j] = rr % 3;
+                rr /= 3;
+            }
+        }
+        float mmid = 3.4e38f, mleft = 3.4e38f, mright = 3.4e38f;
+#pragma unroll 1
+        for (uint32_t qi = 0; qi < nq; qi++) {
+            float rd = 0.0f;
+#pragma unroll
+            for (int j = 1; j < kMaxK; j++) {
+                if ((uint32_t)j >= k) break;
+                const float2 v = rl.gap[qi * kMaxK + j];
+                rd += o[j] == 0 ? v.x : (o[j] == 2 ? v.y : 0.0f);
+            }
+            const float2 v0 = rl.gap[qi * kMaxK];
+            mmid = fminf(mmid, rd);
+            mleft = fminf(mleft, rd + v0.x);
+            mright = fminf(mright, rd + v0.y);
+        }
+        if (r < nrows && mmid <= thr) {
+            uint32_t lo2[kMaxK], hi2[kMaxK];
+            for (uint32_t j = 0; j < k; j++) {
+                lo2[j] = lo[j];
+                hi2[j] = hi[j];
+            }
+            if (!(mleft <= thr) && lo2[0] < c[0]) lo2[0] = c[0];
+            if (!(mright <= thr) && hi2[0] > c[0]) hi2[0] = c[0];
             uint32_t cf, cl;
-            stencil_row(c, lo, hi, r, g, a.ix, &cf, &cl);
+            stencil_row(c, lo2, hi2, r, g, a.ix, &cf, &cl);
             if (cl > cf) {
                 pstart = a.ix.cell_start[cf];
                 plen = a.ix.cell_start[cl] - pstart;
@@ -117,16 +183,81 @@ __device__ __forceinline__ uint32_t cand_pos_adv(const RowList &rl, uint32_t nsl
     return rl.start[slot] + (t - rl.off[slot]);
 }
 
+static_assert(HEGJ_MAX_POINTS < (1u << 29), "SoA byte offsets must fit 32 bits");
+
+// Load from global memory at a wave-uniform base plus a 32-bit byte offset:
+// with the address space explicit this is one global_load with the base in
+// scalar registers and no per-lane 64-bit address arithmetic.
+template <typename T>
+__device__ __forceinline__ T gload_off(const T *base, uint32_t byte_off) {
+    typedef __attribute__((address_space(1))) const char gchar;
+    typedef __attribute__((address_space(1))) const T gT;
+    return *(gT *)((gchar *)base + byte_off);
+}
+
 // Candidate coordinates: D doubles from the cell-ordered SoA copy.
 template <int D>
 __device__ __forceinline__ void load_cand(const JoinArgs &a, uint32_t pos, bool valid, double *x) {
     // unconditional loads (invalid lanes read position 0, always in range, and
     // are masked by the caller): no branch, so the compiler can wait for
     // exactly the chunk it needs instead of every load in flight
-    const uint64_t n = a.g.n;
-    const uint32_t p = valid ? pos : 0u;
+    if constexpr (D <= (int)kAosMaxDim) {
+        // point-major copy: the D coordinates of a candidate are one run of
+        // bytes, and a slot of consecutive candidates one run of lines
+        typedef double gd2 __attribute__((ext_vector_type(2)));
+        typedef __attribute__((address_space(1))) const gd2 gd2g;
+        typedef __attribute__((address_space(1))) const double gdg;
+        const uint64_t p = valid ? pos : 0u;
+        if constexpr (D % 2 == 0) {
+            gd2g *src = (gd2g *)(a.ix.aos + p * D);
 #pragma unroll
-    for (int l = 0; l < D; l++) x[l] = a.ix.soa[(uint64_t)l * n + p];
+            for (int l = 0; l < D / 2; l++) {
+                const gd2 v = src[l];
+                x[2 * l] = v.x;
+                x[2 * l + 1] = v.y;
+            }
+        } else {
+            gdg *src = (gdg *)(a.ix.aos + p * D);
+#pragma unroll
+            for (int l = 0; l < D; l++) x[l] = src[l];
+        }
+    } else {
+        // scalar per-dim base + one 32-bit byte offset for all dims (n <= HEGJ_MAX_POINTS)
+        const uint64_t n = a.g.n;
+        uint32_t off = (valid ? pos : 0u) * (uint32_t)sizeof(double);
+        __asm__("" : "+v"(off));  // an opaque 32-bit offset: the loads take the scalar-base form
+#pragma unroll
+        for (int l = 0; l < D; l++) x[l] = gload_off(a.ix.soa + (uint64_t)l * n, off);
+    }
+}
+
+// Same from the centred FP32 copy (point-major).
+template <int D>
+__device__ __forceinline__ void load_cand32(const JoinArgs &a, uint32_t pos, float *x) {
+    static_assert(D <= (int)kAosMaxDim && D % 2 == 0, "FP32 filter: point-major copy, even D");
+    typedef float gf2 __attribute__((ext_vector_type(2)));
+    typedef __attribute__((address_space(1))) const gf2 gf2g;
+    gf2g *src = (gf2g *)(a.aos32 + (uint64_t)pos * D);
+#pragma unroll
+    for (int l = 0; l < D / 2; l++) {
+        const gf2 v = src[l];
+        x[2 * l] = v.x;
+        x[2 * l + 1] = v.y;
+    }
+}
+
+// v with lane `lane` replaced by the wave-uniform s (v_writelane_b32; the
+// lane select comes from a scalar move, never from a VALU write).
+__device__ __forceinline__ uint32_t set_lane_u32(uint32_t v, uint32_t s, uint32_t lane) {
+    // s is wave-uniform; readfirstlane pins it to a scalar register
+    const uint32_t su = (uint32_t)__builtin_amdgcn_readfirstlane((int)s);
+    __asm__("v_writelane_b32 %0, %1, m0" : "+v"(v) : "s"(su), "{m0}"(lane));
+    return v;
+}
+__device__ __forceinline__ uint64_t set_lane_u64(uint64_t v, uint64_t s, int lane) {
+    const uint32_t lo = set_lane_u32((uint32_t)v, (uint32_t)s, (uint32_t)lane);
+    const uint32_t hi = set_lane_u32((uint32_t)(v >> 32), (uint32_t)(s >> 32), (uint32_t)lane);
+    return ((uint64_t)hi << 32) | lo;
 }
 
 // acc[qi] for one candidate (coordinates in x) against the unit's queries:
@@ -149,11 +280,14 @@ __device__ __forceinline__ void dist_regs(const double *x, const double (*q)[D],
     }
 }
 
-// Same with the query coordinates in LDS (qs[qi * dim + l]).
+// Same with the query coordinates in LDS (qs[qi * dim + l]).  The empty
+// asm with a memory clobber keeps each query's LDS reads next to their use
+// (hoisted, QB * D doubles of them would not fit in registers).
 template <int D, int QB>
 __device__ __forceinline__ void dist_regs_lds(const double *x, const double *qs, double *acc) {
 #pragma unroll
     for (int qi = 0; qi < QB; qi++) {
+        __asm__ volatile("" ::: "memory");
         acc[qi] = 0.0;
 #pragma unroll
         for (int l = 0; l < D; l++) {
@@ -198,19 +332,23 @@ __device__ __forceinline__ void dist_stream(const JoinArgs &a, uint32_t pos, boo
     }
 }
 
-// Query coordinates of a unit kept in scalar registers (small QB * D);
-// otherwise they are staged in LDS, QB * dim doubles per wave.
+// Query coordinates of a unit kept in scalar registers when QB * D doubles
+// leave room for the rest of the kernel's scalar state (48 of the ~100
+// SGPRs; more spills them to VGPR lanes and every use becomes two
+// v_readlane); otherwise they are staged in LDS, QB * dim doubles per wave.
+// The FP32 filter always keeps its QB * D <= 48 floats in SGPRs.
 template <int D, int QB>
 constexpr bool q_in_sgprs() {
-    return D > 0 && D <= 32 && QB * D <= 48;
+    return D > 0 && D <= 32 && QB * D <= 24;
 }
 
-template <int D, int QB>
+template <int D, int QB, bool F32 = false>
 constexpr uint32_t q_lds_bytes(uint32_t dim) {
-    return q_in_sgprs<D, QB>() ? 0u : (uint32_t)(kWavesPerBlockQ * QB * (D > 0 ? D : dim) * sizeof(double));
+    return (F32 || q_in_sgprs<D, QB>()) ? 0u
+                                         : (uint32_t)(kWavesPerBlockQ * QB * (D > 0 ? D : dim) * sizeof(double));
 }
 
-// Hit masks of one unit: mask[qi * nchunks + ch], one 64-bit ballot per
+// Hit masks of one unit: mask[ch * nq + qi], one 64-bit ballot per
 // (query, candidate chunk).  Kept in LDS when they fit, else in a per-wave
 // global scratch slice (written and read back by the same wave, so L2
 // resident), else (very large units) recomputed in the write phase.
@@ -245,17 +383,20 @@ __device__ __forceinline__ bool next_grab(const JoinArgs &a, uint32_t &part, uin
 // argument block, so one captured graph serves every call (the host rewrites
 // the arguments between replays).
 // Waves per SIMD the register budget must allow: 4 (<= 128 VGPRs) for the
-// small-D instances, whose loop is latency-bound and needs occupancy.
-template <int D, int QB>
+// small-D instances with their queries in scalar registers, whose loop is
+// latency-bound and needs occupancy; 2 for the other small-D instances up to
+// 8 queries (queries read from LDS next to their use need more vector
+// registers); 1 above.
+template <int D, int QB, bool F32>
 constexpr int join_min_waves() {
-    return (D > 0 && D <= 8 && QB <= 8) ? 4 : 1;
+    return (D > 0 && D <= 8 && QB <= 8) ? ((F32 || q_in_sgprs<D, QB>()) ? 4 : 2) : 1;
 }
 
 // F32: candidates and queries come from the centred FP32 copy; a pair is
 // accepted when acc32 <= tacc32, rejected when acc32 > trej32, and rechecked
 // with the exact FP64 chain in between (the thresholds bound |acc32 - acc|).
 template <int D, int QB, bool IND = false, bool F32 = false>
-__global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB>())) void k_join(
+__global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())) void k_join(
     JoinArgs av, const JoinArgs *__restrict__ ap) {
     const JoinArgs &a = IND ? *ap : av;
     static_assert(!F32 || (D > 0 && D <= 8 && QB * D <= 48), "FP32 filter: small D only");
@@ -285,11 +426,15 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB>())) voi
             const uint32_t cell = wave_bcast_u32(unit.x), qoff = wave_bcast_u32(unit.y);
             const uint32_t cstart = a.ix.cell_start[cell];
             const uint32_t csize = a.ix.cell_start[cell + 1] - cstart;
-            const uint32_t nq = min((uint32_t)QB, csize - qoff);
-            const uint32_t qpos0 = cstart + qoff;
+            // wave-uniform by construction; readfirstlane lets the compiler
+            // keep them (and the masks derived from them) in scalar registers
+            const uint32_t nq = (uint32_t)__builtin_amdgcn_readfirstlane((int)min((uint32_t)QB, csize - qoff));
+            const uint32_t qpos0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)(cstart + qoff));
             uint32_t nslots;
             bool live;
-            const uint32_t M = build_rows(a, cell, rl, &nslots, &live);
+            const uint32_t M = (uint32_t)__builtin_amdgcn_readfirstlane(
+                (int)build_rows<QB>(a, cell, rl, &nslots, &live, qpos0, nq));
+            nslots = (uint32_t)__builtin_amdgcn_readfirstlane((int)nslots);
             // the unit's query coordinates, once: lane i loads element i
             if constexpr (F32) {
                 constexpr int NQD = QB * DR;
@@ -330,6 +475,8 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB>())) voi
             const bool keep = nmask <= a.gmask_slots || nmask <= (uint32_t)kMaskSlots;
             uint64_t *const hs = nmask <= (uint32_t)kMaskSlots ? hs_lds : hs_glb;
             uint32_t cnt_l = 0;  // lane qi: hits of query qi so far
+            uint64_t mine = 0;   // lane qi: query qi's mask of the current chunk
+            uint32_t nband = 0;  // FP32 filter: (query, chunk) pairs with lanes in the band
 
             uint32_t slot = 0;
             // Candidate coordinates are prefetched kDepth chunks ahead in a
@@ -338,15 +485,12 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB>())) voi
             constexpr int kDepth = kRegs ? (D <= 8 ? 4 : 2) : 1;
             double ring[F32 ? 1 : kDepth][DR];
             float ringf[F32 ? kDepth : 1][DR];
-            uint32_t posr[F32 ? kDepth : 1];
             if constexpr (F32) {
 #pragma unroll
                 for (int p = 0; p < kDepth; p++) {
                     const uint32_t t = p * 64 + lane;
                     const bool v = t < M;
-                    posr[p] = v ? cand_pos_adv(rl, nslots, t, slot) : 0u;
-#pragma unroll
-                    for (int l = 0; l < DR; l++) ringf[p][l] = a.soa32[(uint64_t)l * a.g.n + posr[p]];
+                    load_cand32<DR>(a, v ? cand_pos_adv(rl, nslots, t, slot) : 0u, ringf[p]);
                 }
             } else if (kRegs) {
 #pragma unroll
@@ -360,14 +504,18 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB>())) voi
             // last group have no valid lane (their ballots are empty) and
             // their prefetches read position 0, so the loop body has no
             // branch around a load and the compiler waits for the oldest
-            // chunk only.
+            // chunk only.  Each (query, chunk) ballot goes straight into lane
+            // qi of `mine` (one writelane per half), so no mask stays live in
+            // scalar registers across queries.
             for (uint32_t ch0 = 0; ch0 < nchunks; ch0 += kDepth) {
 #pragma unroll
                 for (int p = 0; p < kDepth; p++) {
                     const uint32_t ch = ch0 + p;
                     if (!kRegs && ch >= nchunks) break;
-                    double acc[QB];
                     const bool valid = ch * 64 + lane < M;
+                    // single compares feed the ballots; validity is applied
+                    // to the scalar masks
+                    const uint64_t vmask = __builtin_amdgcn_ballot_w64(valid);
                     if constexpr (F32) {
                         float accf[QB];
 #pragma unroll
@@ -379,71 +527,102 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB>())) voi
                                 accf[qi] = __builtin_fmaf(dd, dd, accf[qi]);
                             }
                         }
-                        const uint32_t pos = posr[p];
                         {
                             const uint32_t tn = (ch + kDepth) * 64 + lane;
-                            const bool vn = tn < M;
-                            posr[p] = vn ? cand_pos_adv(rl, nslots, tn, slot) : 0u;
-#pragma unroll
-                            for (int l = 0; l < DR; l++) ringf[p][l] = a.soa32[(uint64_t)l * a.g.n + posr[p]];
+                            load_cand32<DR>(a, tn < M ? cand_pos_adv(rl, nslots, tn, slot) : 0u, ringf[p]);
                         }
-                        uint64_t mine = 0;
 #pragma unroll
                         for (int qi = 0; qi < QB; qi++) {
-                            const bool live_q = valid && (uint32_t)qi < nq;
-                            uint64_t m = __ballot(live_q && accf[qi] <= a.tacc32);
-                            const bool band = live_q && !(accf[qi] <= a.tacc32) && !(accf[qi] > a.trej32);
-                            const uint64_t mb = __ballot(band);
+                            const uint64_t vq = (uint32_t)qi < nq ? vmask : 0ull;
+                            const uint64_t m = __builtin_amdgcn_ballot_w64(accf[qi] <= a.tacc32) & vq;
+                            const uint64_t mb = vq & ~m & ~__builtin_amdgcn_ballot_w64(accf[qi] > a.trej32);
                             if (mb) {
-                                // the band: the reference's exact FP64 chain
-                                double e = 0.0;
-#pragma unroll
-                                for (int l = 0; l < DR; l++) {
-                                    const double dd = a.ix.soa[(uint64_t)l * a.g.n + pos] -
-                                                      a.ix.soa[(uint64_t)l * a.g.n + qpos0 + qi];
-                                    e = __builtin_fma(dd, dd, e);
-                                }
-                                m |= __ballot(band && e <= T);
+                                // decided after the loop by the exact FP64 chain
+                                if (lane == 0 && nband < (uint32_t)kBandList) rl.band[nband] = (ch << 4) | (uint32_t)qi;
+                                nband++;
                                 my_band += (uint64_t)__popcll(mb);
                             }
-                            cnt_l += lane == qi ? (uint32_t)__popcll(m) : 0u;
-                            mine = lane == qi ? m : mine;
-                        }
-                        if (keep && (uint32_t)lane < nq && ch < nchunks) hs[lane * nchunks + ch] = mine;
-                        continue;
-                    } else if (kRegs) {
-                        // use ring[p], then refill it with chunk ch + kDepth
-                        // (no copy of registers that still have loads in flight)
-                        if constexpr (kQS) {
-                            dist_regs<DR, QB>(ring[p], qreg, acc);
-                        } else {
-                            // opaque offset: keeps the compiler from hoisting
-                            // QB * D loop-invariant LDS reads into registers
-                            uint32_t qoff = (uint32_t)wid * QB * DR;
-                            __asm__ volatile("" : "+v"(qoff));
-                            dist_regs_lds<DR, QB>(ring[p], q_dyn + qoff, acc);
-                        }
-                        {
-                            const uint32_t tn = (ch + kDepth) * 64 + lane;
-                            const bool vn = tn < M;
-                            load_cand<DR>(a, vn ? cand_pos_adv(rl, nslots, tn, slot) : 0u, vn,
-                                          ring[p]);
+                            mine = set_lane_u64(mine, m, qi);
                         }
                     } else {
-                        const uint32_t t = ch * 64 + lane;
-                        const uint32_t pos = valid ? cand_pos_adv(rl, nslots, t, slot) : 0u;
-                        uint32_t qoff = (uint32_t)wid * QB * dimq;
-                        __asm__ volatile("" : "+v"(qoff));
-                        dist_stream<D, QB>(a, pos, valid, q_dyn + qoff, nq, acc);
-                    }
-                    uint64_t mine = 0;  // lane qi keeps query qi's mask
+                        double acc[QB];
+                        if constexpr (kRegs) {
+                            // use ring[p], then refill it with chunk ch + kDepth
+                            // (no copy of registers that still have loads in flight)
+                            if constexpr (kQS) {
+                                dist_regs<DR, QB>(ring[p], qreg, acc);
+                            } else {
+                                // opaque offset: keeps the compiler from hoisting
+                                // QB * D loop-invariant LDS reads into registers
+                                uint32_t qoff = (uint32_t)wid * QB * DR;
+                                __asm__ volatile("" : "+v"(qoff));
+                                dist_regs_lds<DR, QB>(ring[p], q_dyn + qoff, acc);
+                            }
+                            const uint32_t tn = (ch + kDepth) * 64 + lane;
+                            const bool vn = tn < M;
+                            load_cand<DR>(a, vn ? cand_pos_adv(rl, nslots, tn, slot) : 0u, vn, ring[p]);
+                        } else {
+                            const uint32_t t = ch * 64 + lane;
+                            const uint32_t pos = valid ? cand_pos_adv(rl, nslots, t, slot) : 0u;
+                            uint32_t qoff = (uint32_t)wid * QB * dimq;
+                            __asm__ volatile("" : "+v"(qoff));
+                            dist_stream<D, QB>(a, pos, valid, q_dyn + qoff, nq, acc);
+                        }
 #pragma unroll
-                    for (int qi = 0; qi < QB; qi++) {
-                        const uint64_t m = __ballot(valid && (uint32_t)qi < nq && acc[qi] <= T);
-                        cnt_l += lane == qi ? (uint32_t)__popcll(m) : 0u;
-                        mine = lane == qi ? m : mine;
+                        for (int qi = 0; qi < QB; qi++)
+                            mine = set_lane_u64(mine, __builtin_amdgcn_ballot_w64(acc[qi] <= T) &
+                                                          ((uint32_t)qi < nq ? vmask : 0ull), qi);
                     }
-                    if (keep && (uint32_t)lane < nq && ch < nchunks) hs[lane * nchunks + ch] = mine;
+                    cnt_l += (uint32_t)__popcll(mine);
+                    if (keep && (uint32_t)lane < nq && ch < nchunks) hs[ch * nq + lane] = mine;
+                }
+            }
+            if constexpr (F32) {
+                // The band: the reference's exact FP64 chain decides the lanes
+                // whose FP32 sum lies between the two thresholds.  Listed
+                // (query, chunk) pairs, or all of them when the list overflowed.
+                if (nband) {
+                    wave_lds_sync();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                    const bool all = nband > (uint32_t)kBandList;
+                    const uint32_t nitems = all ? nq * nchunks : nband;
+                    const uint64_t n = a.g.n;
+                    for (uint32_t e = 0; e < nitems; e++) {
+                        uint32_t qi, ch;
+                        if (all) {
+                            ch = e / nq;
+                            qi = e - ch * nq;
+                        } else {
+                            const uint32_t v = rl.band[e];
+                            qi = v & 15u;
+                            ch = v >> 4;
+                        }
+                        const uint32_t t = ch * 64 + lane;
+                        const bool valid = t < M;
+                        const uint32_t pos = valid ? cand_pos(rl, nslots, t) : 0u;
+                        float af = 0.0f;
+#pragma unroll
+                        for (int l = 0; l < DR; l++) {
+                            const float dd = a.soa32[(uint64_t)l * n + pos] - a.soa32[(uint64_t)l * n + qpos0 + qi];
+                            af = __builtin_fmaf(dd, dd, af);
+                        }
+                        const bool band = valid && !(af <= a.tacc32) && !(af > a.trej32);
+                        bool hit = false;
+                        if (band) {
+                            double e64 = 0.0;
+#pragma unroll
+                            for (int l = 0; l < DR; l++) {
+                                const double dd = a.ix.soa[(uint64_t)l * n + pos] - a.ix.soa[(uint64_t)l * n + qpos0 + qi];
+                                e64 = __builtin_fma(dd, dd, e64);
+                            }
+                            hit = e64 <= T;
+                        }
+                        const uint64_t madd = __ballot(hit);
+                        if (madd) {
+                            if (keep && lane == 0) hs[ch * nq + qi] |= madd;
+                            cnt_l += (uint32_t)lane == qi ? (uint32_t)__popcll(madd) : 0u;
+                        }
+                    }
                 }
             }
             my_evals += (uint64_t)M * nq;
@@ -483,50 +662,73 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB>())) voi
             // the unit's queries; lane qi keeps query qi's write cursor, and
             // each query's neighbours come out in candidate (cell) order.
             uint64_t o_l = base + excl_l;
-            uint32_t slot2 = 0;
-            for (uint32_t ch = 0; ch < nchunks; ch++) {
-                const uint32_t t = ch * 64 + lane;
-                const bool valid = t < M;
-                const uint32_t pos = valid ? cand_pos_adv(rl, nslots, t, slot2) : 0u;
-                const uint32_t id = valid ? a.ix.ila[pos] : 0u;
-                double x[DR];
-                if (!keep) {
-                    // masks did not fit anywhere (very large units): evaluate
-                    // the chunk again, one query at a time
-                    if constexpr (kRegs) load_cand<DR>(a, pos, valid, x);
-                    my_evals += (uint64_t)min(64u, M - ch * 64) * nq;
-                }
+            // The loads each chunk depends on -- its neighbour ids
+            // (indexLookupArr) and, when kept, its masks (lane qi: query qi's)
+            // -- are issued kW chunks ahead, so no iteration waits on a
+            // fresh global load.
+            constexpr int kW = 4;
+            uint32_t idr[kW];
+            uint64_t mkr[kW];
+            uint32_t slot2 = 0, slot3 = 0;
+            auto fetch = [&](uint32_t c, uint32_t &id, uint64_t &mk) {
+                const uint32_t t = c * 64 + lane;
+                id = a.ix.ila[t < M ? cand_pos_adv(rl, nslots, t, slot2) : 0u];
+                mk = (keep && c < nchunks && (uint32_t)lane < nq) ? hs[c * nq + lane] : 0ull;
+            };
 #pragma unroll
-                for (int qi = 0; qi < QB; qi++) {
-                    uint64_t m = 0;
-                    if ((uint32_t)qi >= nq) {
-                    } else if (keep) {
-                        m = hs[qi * nchunks + ch];
-                    } else {
-                        double acc1 = 0.0;
-                        if constexpr (kRegs) {
+            for (int p = 0; p < kW; p++) fetch((uint32_t)p, idr[p], mkr[p]);
+            for (uint32_t ch0 = 0; ch0 < nchunks; ch0 += kW) {
 #pragma unroll
-                            for (int l = 0; l < DR; l++) {
-                                const double q = F32 ? a.ix.soa[(uint64_t)l * a.g.n + qpos0 + qi]
-                                                 : kQS ? qreg[kQS && !F32 ? qi : 0][kQS && !F32 ? l : 0]
-                                                       : qs[qi * DR + l];
-                                const double dd = x[l] - q;
-                                acc1 = __builtin_fma(dd, dd, acc1);
-                            }
+                for (int p = 0; p < kW; p++) {
+                    const uint32_t ch = ch0 + p;
+                    if (ch >= nchunks) break;
+                    const uint32_t id = idr[p];
+                    const uint64_t mk = mkr[p];
+                    fetch(ch + kW, idr[p], mkr[p]);
+                    const uint32_t t = ch * 64 + lane;
+                    const bool valid = t < M;
+                    uint32_t pos = 0;
+                    double x[DR];
+                    if (!keep) {
+                        // masks did not fit anywhere (very large units): evaluate
+                        // the chunk again, one query at a time
+                        pos = valid ? cand_pos_adv(rl, nslots, t, slot3) : 0u;
+                        if constexpr (kRegs) load_cand<DR>(a, pos, valid, x);
+                        my_evals += (uint64_t)min(64u, M - ch * 64) * nq;
+                    }
+#pragma unroll
+                    for (int qi = 0; qi < QB; qi++) {
+                        uint64_t m = 0;
+                        if ((uint32_t)qi >= nq) {
+                        } else if (keep) {
+                            m = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(mk >> 32), qi) << 32) |
+                                (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)mk, qi);
                         } else {
-                            dist_stream<D, 1>(a, pos, valid, qs + qi * dimq, 1, &acc1);
+                            double acc1 = 0.0;
+                            if constexpr (kRegs) {
+#pragma unroll
+                                for (int l = 0; l < DR; l++) {
+                                    const double q = F32 ? a.ix.soa[(uint64_t)l * a.g.n + qpos0 + qi]
+                                                     : kQS ? qreg[kQS && !F32 ? qi : 0][kQS && !F32 ? l : 0]
+                                                           : qs[qi * DR + l];
+                                    const double dd = x[l] - q;
+                                    acc1 = __builtin_fma(dd, dd, acc1);
+                                }
+                            } else {
+                                dist_stream<D, 1>(a, pos, valid, qs + qi * dimq, 1, &acc1);
+                            }
+                            m = __ballot(valid && acc1 <= T);
                         }
-                        m = __ballot(valid && acc1 <= T);
+                        if (m == 0) continue;
+                        const uint64_t oq = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(o_l >> 32), qi) << 32) |
+                                            __builtin_amdgcn_readlane((uint32_t)o_l, qi);
+                        const uint32_t qid = __builtin_amdgcn_readlane(qid_l, qi);
+                        if ((m >> lane) & 1ull) {
+                            a.out[oq + lanes_below(m)] = id;
+                            my_hash += mix64(((uint64_t)qid << 32) | id);
+                        }
+                        o_l += lane == qi ? (uint64_t)__popcll(m) : 0ull;
                     }
-                    if (m == 0) continue;
-                    const uint64_t oq = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(o_l >> 32), qi) << 32) |
-                                        __builtin_amdgcn_readlane((uint32_t)o_l, qi);
-                    const uint32_t qid = __builtin_amdgcn_readlane(qid_l, qi);
-                    if ((m >> lane) & 1ull) {
-                        a.out[oq + lanes_below(m)] = id;
-                        my_hash += mix64(((uint64_t)qid << 32) | id);
-                    }
-                    o_l += lane == qi ? (uint64_t)__popcll(m) : 0ull;
                 }
             }
         }
@@ -631,7 +833,7 @@ constexpr bool f32_inst() {
 template <int D, int QB>
 static hegj_status launch_join_t(const JoinArgs &a, const JoinArgs *ap, uint32_t blocks,
                                  hipStream_t s) {
-    const uint32_t lds = q_lds_bytes<D, QB>(a.g.dim);
+    const uint32_t lds = a.f32 ? q_lds_bytes<D, QB, true>(a.g.dim) : q_lds_bytes<D, QB>(a.g.dim);
     if (lds > kMaxQueryLds)
         return fail(HEGJ_ERR_UNSUPPORTED, "queries_per_unit * dim too large for LDS; lower queries_per_unit");
     if (a.f32) {

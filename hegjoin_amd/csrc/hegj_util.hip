@@ -371,9 +371,17 @@ __global__ __launch_bounds__(256) void k_soa32(const double *__restrict__ soa, u
     if (i < total) out[i] = (float)(soa[i] - centre[i / n]);
 }
 
+// aos32[p * dim + l] = soa32[l * n + p] (the same FP32 values, point-major).
+__global__ __launch_bounds__(256) void k_aos32(const float *__restrict__ soa32, uint32_t n, uint32_t dim,
+                                               float *__restrict__ aos32) {
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= n) return;
+    for (uint32_t l = 0; l < dim; l++) aos32[(uint64_t)p * dim + l] = soa32[(uint64_t)l * n + p];
+}
+
 }  // namespace
 
-hegj_status make_soa32(const GridGeom &g, const DevIndex &ix, float **soa32, double *radius,
+hegj_status make_soa32(const GridGeom &g, const DevIndex &ix, float **soa32, float **aos32, double *radius,
                        double **centre_dev, hipStream_t s) {
     const uint32_t n = g.n, dim = g.dim;
     const uint32_t blocks = std::min<uint32_t>(ceil_div(n, 256), 512);
@@ -403,6 +411,11 @@ hegj_status make_soa32(const GridGeom &g, const DevIndex &ix, float **soa32, dou
     const uint64_t total = (uint64_t)n * dim;
     k_soa32<<<ceil_div(total, 256), 256, 0, s>>>(ix.soa, total, n, centre, *soa32);
     HIP_TRY(hipGetLastError());
+    if (aos32 && dim <= kAosMaxDim) {
+        HIP_TRY(hipMalloc(aos32, sizeof(float) * (size_t)n * dim));
+        k_aos32<<<ceil_div(n, 256), 256, 0, s>>>(*soa32, n, dim, *aos32);
+        HIP_TRY(hipGetLastError());
+    }
     HIP_TRY(hipStreamSynchronize(s));
     return HEGJ_OK;
 }

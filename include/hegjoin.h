@@ -44,6 +44,9 @@ extern "C" {
 #endif
 
 #define HEGJ_MAX_INDEXED_DIMS 6
+/* Largest point count of an index: each dimension's coordinate array stays
+ * under 4 GiB, so the join kernels address it with 32-bit byte offsets. */
+#define HEGJ_MAX_POINTS 536870911u
 
 typedef enum {
     HEGJ_OK = 0,
@@ -182,6 +185,10 @@ typedef struct {
     uint32_t part_by_block;   /* 0: range by hardware XCD id, 1: by workgroup index */
     uint32_t use_graph;       /* FP64 refine: keep the plan in the index and run the
                                  batch as a replay of a captured hipGraph */
+    uint32_t no_prune;        /* 1: evaluate every point of every stencil cell.  0
+                                 (default): skip stencil cells whose box lies farther
+                                 than eps from all queries of a work unit, with
+                                 rounding margins; the pair set is the same */
 } hegj_join_opts;
 
 void hegj_join_opts_default(hegj_join_opts *opts);

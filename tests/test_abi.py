@@ -54,6 +54,7 @@ int main(void) {
     S(hegj_index_opts) S(hegj_index_info) S(hegj_join_opts) S(hegj_result_info)
     S(hegj_serve_opts) S(hegj_request_info) S(hegj_serve_stats)
     O(hegj_index_info, build_ms) O(hegj_join_opts, part_by_block) O(hegj_join_opts, use_graph)
+    O(hegj_join_opts, no_prune)
     O(hegj_result_info, refine_used) O(hegj_result_info, graph_replay) O(hegj_serve_opts, result_bytes)
     O(hegj_serve_opts, device) O(hegj_request_info, latency_ms) O(hegj_request_info, band_rechecks)
     O(hegj_request_info, refine_used) O(hegj_serve_stats, wall_ms)

@@ -352,3 +352,71 @@ def test_cli_end_to_end(H, oracle, tmp_path):
     off = np.frombuffer(raw[4:4 + 8 * (n + 1)], np.uint64)
     nb = np.frombuffer(raw[4 + 8 * (n + 1):], np.uint32)
     assert np.array_equal(off, eoff) and np.array_equal(nb, enb)
+
+
+def lattice_boundary(dim, eps, seed, offset=0.0, cmax=30):
+    """Points on the cell lattice and a hair either side of it (cell
+    boundaries at min - eps + c eps), and pairs at eps (1 +- 1e-15, 0)
+    straddling cells along one or several dims."""
+    rng = np.random.default_rng(seed)
+    c = rng.integers(2, cmax, size=(1500, dim)).astype(np.float64)
+    lat = offset + eps * c
+    jit = lat * (1 + rng.choice([-1e-16, 0.0, 1e-16, -2e-16, 2e-16], size=lat.shape))
+    d = np.zeros((1500, dim))
+    axes = rng.integers(0, dim, size=(1500, 2))
+    d[np.arange(1500), axes[:, 0]] = 1.0
+    d[np.arange(1500), axes[:, 1]] += 1.0
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rel = rng.choice([-1e-15, 0.0, 1e-15], 1500)
+    part = jit + d * (eps * (1 + rel))[:, None]
+    return np.ascontiguousarray(np.concatenate([lat, jit, part]))
+
+
+@pytest.mark.parametrize("dim,k,offset,cmax", [
+    (2, 2, 0.0, 30), (3, 3, 0.0, 30), (6, 6, 0.0, 30), (6, 4, 0.0, 30), (8, 6, 0.0, 30),
+    (6, 6, 3.0e5, 30), (2, 2, 1.0e7, 30), (2, 2, 0.0, 10 ** 7), (3, 3, 0.0, 10 ** 6)])
+def test_pruning_is_exact(H, oracle, dim, k, offset, cmax):
+    """Cell-box pruning leaves out stencil cells farther than eps from every
+    query of a unit; on points on and next to cell boundaries (also far from
+    the origin, and at cell coordinates up to 1e7) the pair set is the
+    oracle's, with and without pruning, for the FP64 and FP32 paths."""
+    eps = 0.01
+    pts = lattice_boundary(dim, eps, seed=70 + dim + k, offset=offset, cmax=cmax)
+    eoff, enb = oracle.join(pts, k, eps)
+    refines = ["fp64", "f32"] if (dim in (2, 6) and offset == 0.0 and cmax == 30) else ["fp64"]
+    for refine in refines:
+        for prune in (True, False):
+            for qb in (4, 8):
+                print("case dim", dim, "k", k, "offset", offset, refine, "prune", prune, "qb", qb, flush=True)
+                off, nb, info = gpu_join(H, pts, eps, k, refine=refine, prune=prune, queries_per_unit=qb)
+                assert np.array_equal(off, eoff) and np.array_equal(nb, enb)
+
+
+def test_pruning_reduces_evaluations(H, oracle):
+    """On uniform data with a few points per cell, pruning evaluates fewer
+    candidates than the full stencil and finds the same pairs."""
+    pts = oracle.generate_expo(40000, 6, seed=77)
+    eps = 0.02
+    a = gpu_join(H, pts, eps, 6, refine="fp64", queries_per_unit=4, prune=True)
+    b = gpu_join(H, pts, eps, 6, refine="fp64", queries_per_unit=4, prune=False)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert a[2]["evals_executed"] < b[2]["evals_executed"]
+    assert b[2]["evals_executed"] == b[2]["total_candidates"]
+
+
+@pytest.mark.parametrize("qb", [4, 8])
+def test_f32_band_list_overflow(H, oracle, qb):
+    """A cluster of near-identical points with partners at eps (1 +- 1e-15):
+    nearly every (query, chunk) pair of a unit has lanes in the FP32 band,
+    more than the per-wave list holds, so every pair is rechecked in FP64."""
+    rng = np.random.default_rng(81)
+    eps = 0.01
+    base = 0.5 + 1e-13 * rng.random((1200, 6))
+    d = rng.standard_normal((1200, 6))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rel = rng.choice([-1e-15, 0.0, 1e-15], 1200)
+    pts = np.ascontiguousarray(np.concatenate([base, base + d * (eps * (1 + rel))[:, None]]))
+    off, nb, info = gpu_join(H, pts, eps, 6, refine="f32", queries_per_unit=qb)
+    eoff, enb = oracle.join(pts, 6, eps)
+    assert info["refine_used"] == "f32" and info["band_rechecks"] > 64 * 64
+    assert np.array_equal(off, eoff) and np.array_equal(nb, enb)
