@@ -67,9 +67,10 @@ template <int QB>
 __device__ uint32_t build_rows(const JoinArgs &a, uint32_t cell, RowList &rl, uint32_t *nslots,
                                bool *live, uint32_t qpos0, uint32_t nq) {
     const GridGeom &g = a.g;
+    const DevIndex &ix = a.ix;
     const int lane = threadIdx.x & 63;
     uint32_t c[kMaxK], lo[kMaxK], hi[kMaxK];
-    decode_cell(a.ix.cell_key[cell], g, c);
+    decode_cell(ix.cell_key[cell], g, c);
     bool ok = true;
     for (uint32_t j = 0; j < g.k; j++) ok &= stencil_dim(c[j], g.nCells[j], g.emulate_cell0, &lo[j], &hi[j]);
     *live = ok;
@@ -87,7 +88,7 @@ __device__ uint32_t build_rows(const JoinArgs &a, uint32_t cell, RowList &rl, ui
 #pragma unroll
         for (int jj = 0; jj < kMaxK; jj++)
             if ((uint32_t)jj == j) cj = c[jj];
-        const double x = a.ix.soa[(uint64_t)j * g.n + qpos0 + qi];
+        const double x = ix.soa[(uint64_t)j * g.n + qpos0 + qi];
         double f = (x - g.minArr[j]) / g.eps - (double)cj;
         f = f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
         const double mg = 1e-6 + 2e-15 * ((double)cj + 2.0);
@@ -97,20 +98,41 @@ __device__ uint32_t build_rows(const JoinArgs &a, uint32_t cell, RowList &rl, ui
     wave_lds_sync();
     const float thr = a.prune ? (float)(g.thresh / (g.eps * g.eps)) * (1.0f + 1e-5f) : INFINITY;
     const uint32_t nrows = pow3(k - 1);
-    uint32_t slots = 0, total = 0;
-    for (uint32_t r0 = 0; r0 < nrows; r0 += 64) {
-        const uint32_t r = r0 + lane;
-        uint32_t pstart = 0, plen = 0;
-        // offset digit of this row along indexed dims 1..k-1 (0: -1, 1: 0, 2: +1)
-        uint32_t o[kMaxK];
-        {
-            uint32_t rr = r;
+    // Lane l takes rows l, l + 64, l + 128, l + 192: each dependent step
+    // below (hash probe, row bounds, dim-0 window, cell bounds) issues the
+    // loads of all its rows at once, so a unit waits for ~6 round trips to
+    // memory instead of ~6 per group of 64 rows.
+    constexpr int R = (kMaxRows + 63) / 64;
+    uint64_t rkey[R];
+    uint32_t wlo[R], whi[R], slot[R];
+    bool want[R];
 #pragma unroll
-            for (int j = 1; j < kMaxK; j++) {
-                o[j] = rr % 3;
-                rr /= 3;
+    for (int i = 0; i < R; i++) {
+        const uint32_t r = (uint32_t)lane + 64u * i;
+        want[i] = false;
+        rkey[i] = 0;
+        wlo[i] = lo[0];
+        whi[i] = hi[0];
+        if (r >= nrows) continue;
+        // offset digits of row r along dims 1..k-1 (0: -1, 1: 0, 2: +1)
+        uint32_t o[kMaxK];
+        uint32_t rr = r;
+        bool inside = true;
+        uint64_t key = 0, mult = 1;
+#pragma unroll
+        for (int j = 1; j < kMaxK; j++) {
+            o[j] = rr % 3;
+            rr /= 3;
+            if ((uint32_t)j < k) {
+                const uint32_t cj = c[j] + o[j] - 1;  // wraps for c[j] == 0, rejected here
+                inside &= cj >= lo[j] && cj <= hi[j];
+                key += (uint64_t)cj * mult;
+                mult *= g.nCells[j];
             }
         }
+        if (!inside) continue;
+        // cell-box pruning: the row's squared box distance to the nearest
+        // query (mid: dim-0 offset 0; left / right: offset -1 / +1)
         float mmid = 3.4e38f, mleft = 3.4e38f, mright = 3.4e38f;
 #pragma unroll 1
         for (uint32_t qi = 0; qi < nq; qi++) {
@@ -126,33 +148,91 @@ __device__ uint32_t build_rows(const JoinArgs &a, uint32_t cell, RowList &rl, ui
             mleft = fminf(mleft, rd + v0.x);
             mright = fminf(mright, rd + v0.y);
         }
-        if (r < nrows && mmid <= thr) {
-            uint32_t lo2[kMaxK], hi2[kMaxK];
-            for (uint32_t j = 0; j < k; j++) {
-                lo2[j] = lo[j];
-                hi2[j] = hi[j];
-            }
-            if (!(mleft <= thr) && lo2[0] < c[0]) lo2[0] = c[0];
-            if (!(mright <= thr) && hi2[0] > c[0]) hi2[0] = c[0];
-            uint32_t cf, cl;
-            stencil_row(c, lo2, hi2, r, g, a.ix, &cf, &cl);
-            if (cl > cf) {
-                pstart = a.ix.cell_start[cf];
-                plen = a.ix.cell_start[cl] - pstart;
-            }
+        if (!(mmid <= thr)) continue;
+        if (!(mleft <= thr) && wlo[i] < c[0]) wlo[i] = c[0];
+        if (!(mright <= thr) && whi[i] > c[0]) whi[i] = c[0];
+        want[i] = true;
+        rkey[i] = key;
+    }
+    // hash probes: first slots together, collisions (rare) one by one
+    uint64_t hk[R];
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+        slot[i] = hash_slot(rkey[i]) & ix.hmask;
+        hk[i] = want[i] ? ix.hkeys[slot[i]] : ~0ull;
+    }
+    uint32_t row[R];
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+        uint32_t sl = slot[i];
+        uint64_t q = hk[i];
+        while (want[i] && q != rkey[i] && q != ~0ull) {
+            sl = (sl + 1) & ix.hmask;
+            q = ix.hkeys[sl];
         }
-        uint64_t nz = __ballot(plen > 0);
-        // inclusive scan of plen across the wave
-        uint32_t x = plen;
+        slot[i] = sl;
+        want[i] = want[i] && q == rkey[i];
+    }
+#pragma unroll
+    for (int i = 0; i < R; i++) row[i] = want[i] ? ix.hvals[slot[i]] : 0u;
+    uint32_t rs[R], re[R];
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+        rs[i] = want[i] ? ix.row_start[row[i]] : 0u;
+        re[i] = want[i] ? ix.row_start[row[i] + 1] : 0u;
+    }
+    // dim-0 window [wlo, whi] inside the row: rows of up to 4 cells compare
+    // their keys loaded together, longer rows binary-search
+    constexpr int kShort = 4;
+    uint32_t ca[R], cb[R];
+    uint64_t ck[R][kShort];
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+        const uint32_t len = re[i] - rs[i];
+#pragma unroll
+        for (int j = 0; j < kShort; j++)
+            ck[i][j] = (len > 0 && len <= (uint32_t)kShort) ? ix.cell_key[rs[i] + min((uint32_t)j, len - 1)] : 0ull;
+    }
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+        const uint32_t len = re[i] - rs[i];
+        const uint64_t base = rkey[i] * g.nCells[0];
+        uint32_t na = 0, nb = 0;
+#pragma unroll
+        for (int j = 0; j < kShort; j++) {
+            na += ((uint32_t)j < len && ck[i][j] < base + wlo[i]) ? 1u : 0u;
+            nb += ((uint32_t)j < len && ck[i][j] <= base + whi[i]) ? 1u : 0u;
+        }
+        ca[i] = rs[i] + na;
+        cb[i] = rs[i] + nb;
+        if (len > (uint32_t)kShort) {
+            ca[i] = lower_bound_u64(ix.cell_key, rs[i], re[i], base + wlo[i]);
+            cb[i] = lower_bound_u64(ix.cell_key, ca[i], re[i], base + whi[i] + 1);
+        }
+    }
+    uint32_t pstart[R], plen[R];
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+        const bool any = cb[i] > ca[i];
+        pstart[i] = any ? ix.cell_start[ca[i]] : 0u;
+        plen[i] = any ? ix.cell_start[cb[i]] - pstart[i] : 0u;
+    }
+    // compact the non-empty rows, in row order, into the LDS list
+    uint32_t slots = 0, total = 0;
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+        if (64u * i >= nrows) break;
+        const uint64_t nz = __ballot(plen[i] > 0);
+        uint32_t x = plen[i];
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
-            uint32_t y = __shfl_up(x, o, 64);
+            const uint32_t y = __shfl_up(x, o, 64);
             if (lane >= o) x += y;
         }
-        if (plen > 0) {
-            uint32_t s = slots + lanes_below(nz);
-            rl.start[s] = pstart;
-            rl.off[s] = total + x - plen;
+        if (plen[i] > 0) {
+            const uint32_t s = slots + lanes_below(nz);
+            rl.start[s] = pstart[i];
+            rl.off[s] = total + x - plen[i];
         }
         slots += (uint32_t)__popcll(nz);
         total += __shfl(x, 63, 64);
@@ -666,7 +746,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())
             // (indexLookupArr) and, when kept, its masks (lane qi: query qi's)
             // -- are issued kW chunks ahead, so no iteration waits on a
             // fresh global load.
-            constexpr int kW = 4;
+            constexpr int kW = 8;
             uint32_t idr[kW];
             uint64_t mkr[kW];
             uint32_t slot2 = 0, slot3 = 0;
