@@ -453,6 +453,7 @@ hegj_status run_batches(hegj_index *h, const hegj_join_opts &opts, uint32_t qb, 
             join_ms += t.stop();
             launches++;
         }
+        HEGJ_TRY(launch_pair_hash(a, capacity, s));
         HIP_TRY(hipMemcpyAsync(&c, P.d_ctr, sizeof(c), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         if (r->batches.size() >= (size_t)kMaxBatches && c.n_deferred)
@@ -546,6 +547,7 @@ hegj_status capture_graph(hegj_index *h, GraphPlan &G) {
         st = fail(HEGJ_ERR_HIP, "graph capture: argument upload");
     if (st == HEGJ_OK) st = zero_words(G.P.d_ctr, sizeof(Counters), s);
     if (st == HEGJ_OK) st = launch_join(proto, G.qb, G.blocks, s, G.d_args);
+    if (st == HEGJ_OK) st = launch_pair_hash(proto, G.P.capacity, s, G.d_args);
     if (st == HEGJ_OK &&
         hipMemcpyAsync(G.h_ctr, G.P.d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, s) != hipSuccess)
         st = fail(HEGJ_ERR_HIP, "graph capture: counter download");

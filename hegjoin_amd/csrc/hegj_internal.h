@@ -159,6 +159,11 @@ hegj_status build_units(const GridGeom &g, const DevIndex &ix, uint32_t qb, cons
 // `a` then only selects the instantiation by a.g.dim).
 hegj_status launch_join(const JoinArgs &a, uint32_t qb, uint32_t blocks, hipStream_t s,
                         const JoinArgs *dev_args = nullptr);
+// Adds the pair hash of the batch's finished rows (a.out, a.row_off,
+// a.row_cnt, a.out_base, a.capacity) to *a.hash; pairs_hint (expected pairs)
+// picks the lanes per row.
+hegj_status launch_pair_hash(const JoinArgs &a, uint64_t pairs_hint, hipStream_t s,
+                             const JoinArgs *dev_args = nullptr);
 // FP32 filter support: centred float copy of the cell-ordered coordinates
 // (SoA, and point-major into *aos32 when aos32 != nullptr and dim <=
 // kAosMaxDim) and the largest |x - centre| (hegj_util.hip).

@@ -265,6 +265,17 @@ __device__ __forceinline__ uint32_t cand_pos_adv(const RowList &rl, uint32_t nsl
 
 static_assert(HEGJ_MAX_POINTS < (1u << 29), "SoA byte offsets must fit 32 bits");
 
+// Plain global loads and stores (the explicit address space gives global_*
+// instructions: flat ones would also count against LDS waits).
+template <typename T>
+__device__ __forceinline__ T gld(const T *p) {
+    return *(__attribute__((address_space(1))) const T *)p;
+}
+template <typename T>
+__device__ __forceinline__ void gst(T *p, T v) {
+    *(__attribute__((address_space(1))) T *)p = v;
+}
+
 // Load from global memory at a wave-uniform base plus a 32-bit byte offset:
 // with the address space explicit this is one global_load with the base in
 // scalar registers and no per-lane 64-bit address arithmetic.
@@ -490,7 +501,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())
     const double T = a.g.thresh;
     constexpr bool kRegs = (D > 0 && D <= kRegDims);
     constexpr int DR = kRegs ? D : 1;
-    uint64_t my_hash = 0, my_evals = 0, my_pairs = 0;
+    uint64_t my_evals = 0, my_pairs = 0;
     constexpr bool kQS = q_in_sgprs<D, QB>();
     const uint32_t dimq = D > 0 ? (uint32_t)D : a.g.dim;
     double *const qs = q_dyn + (size_t)wid * QB * dimq;
@@ -739,9 +750,20 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())
             }
             if (total == 0) continue;
             // Chunk-major write: one coalesced id load per chunk serves all of
-            // the unit's queries; lane qi keeps query qi's write cursor, and
-            // each query's neighbours come out in candidate (cell) order.
-            uint64_t o_l = base + excl_l;
+            // the unit's queries, each query's neighbours come out in
+            // candidate (cell) order.  The write cursors are wave-uniform
+            // (scalar registers), the hit lanes of a mask store directly
+            // under that mask.  (The pair hash is a separate pass over the
+            // finished rows: here only a few lanes of each instruction would
+            // be active.)
+            uint64_t oq[QB];
+            {
+                const uint64_t o_l = base + excl_l;
+#pragma unroll
+                for (int qi = 0; qi < QB; qi++)
+                    oq[qi] = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(o_l >> 32), qi) << 32) |
+                             (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)o_l, qi);
+            }
             // The loads each chunk depends on -- its neighbour ids
             // (indexLookupArr) and, when kept, its masks (lane qi: query qi's)
             // -- are issued kW chunks ahead, so no iteration waits on a
@@ -752,7 +774,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())
             uint32_t slot2 = 0, slot3 = 0;
             auto fetch = [&](uint32_t c, uint32_t &id, uint64_t &mk) {
                 const uint32_t t = c * 64 + lane;
-                id = a.ix.ila[t < M ? cand_pos_adv(rl, nslots, t, slot2) : 0u];
+                id = gld(a.ix.ila + (t < M ? cand_pos_adv(rl, nslots, t, slot2) : 0u));
                 mk = (keep && c < nchunks && (uint32_t)lane < nq) ? hs[c * nq + lane] : 0ull;
             };
 #pragma unroll
@@ -800,24 +822,16 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())
                             m = __ballot(valid && acc1 <= T);
                         }
                         if (m == 0) continue;
-                        const uint64_t oq = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(o_l >> 32), qi) << 32) |
-                                            __builtin_amdgcn_readlane((uint32_t)o_l, qi);
-                        const uint32_t qid = __builtin_amdgcn_readlane(qid_l, qi);
-                        if ((m >> lane) & 1ull) {
-                            a.out[oq + lanes_below(m)] = id;
-                            my_hash += mix64(((uint64_t)qid << 32) | id);
-                        }
-                        o_l += lane == qi ? (uint64_t)__popcll(m) : 0ull;
+                        if (__builtin_amdgcn_inverse_ballot_w64(m)) gst(a.out + oq[qi] + lanes_below(m), id);
+                        oq[qi] += (uint64_t)__popcll(m);
                     }
                 }
             }
         }
     }
     // one atomic per wave for the counters
-    for (int o = 32; o > 0; o >>= 1) my_hash += __shfl_xor(my_hash, o, 64);
     if (lane == 0) {
         if (F32 && my_band && a.band) atomicAdd(a.band, (unsigned long long)my_band);
-        if (my_hash) atomicAdd(a.hash, (unsigned long long)my_hash);
         if (my_evals) atomicAdd(a.evals, (unsigned long long)my_evals);
         if (my_pairs) atomicAdd(a.pairs, (unsigned long long)my_pairs);
     }
@@ -958,6 +972,57 @@ hegj_status launch_join(const JoinArgs &a, uint32_t qb, uint32_t blocks, hipStre
         case 16: return launch_join_qb<16>(a, dev_args, blocks, s);
         default: return fail(HEGJ_ERR_INVALID_ARG, "queries_per_unit must be 4, 8 or 16");
     }
+}
+
+// Pair hash of one batch: sum of mix64(query << 32 | neighbour) over the rows
+// the batch holds (row offsets in [out_base, out_base + capacity)).  G lanes
+// per row (G = 64 for long rows), each reading 4 entries per step, so a row
+// is read in runs of 4G entries; one atomic per wave.
+template <bool IND, int G>
+__global__ __launch_bounds__(256) void k_pair_hash(JoinArgs av, const JoinArgs *__restrict__ ap) {
+    const JoinArgs &a = IND ? *ap : av;
+    const uint32_t n = a.g.n;
+    const uint32_t sub = threadIdx.x & (G - 1);
+    const uint32_t grp = (blockIdx.x * 256 + threadIdx.x) / G, ngrp = (gridDim.x * 256) / G;
+    uint64_t h = 0;
+    for (uint32_t q = grp; q < n; q += ngrp) {
+        const uint32_t cnt = gld(a.row_cnt + q);
+        if (cnt == 0) continue;
+        const uint64_t off = gld(a.row_off + q);
+        if (off < a.out_base || off - a.out_base >= a.capacity) continue;
+        const uint32_t *row = a.out + (off - a.out_base);
+        const uint64_t qk = (uint64_t)q << 32;
+        uint32_t j = sub;
+        for (; j + 3 * G < cnt; j += 4 * G) {
+            const uint32_t v0 = gld(row + j), v1 = gld(row + j + G), v2 = gld(row + j + 2 * G),
+                           v3 = gld(row + j + 3 * G);
+            h += mix64(qk | v0) + mix64(qk | v1) + mix64(qk | v2) + mix64(qk | v3);
+        }
+        for (; j < cnt; j += G) h += mix64(qk | gld(row + j));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) h += __shfl_xor(h, o, 64);
+    if ((threadIdx.x & 63) == 0 && h) atomicAdd(a.hash, (unsigned long long)h);
+}
+
+hegj_status launch_pair_hash(const JoinArgs &a, uint64_t pairs_hint, hipStream_t s, const JoinArgs *dev_args) {
+    const uint32_t n = std::max<uint32_t>(a.g.n, 1);
+    const bool longrows = pairs_hint / n >= 256;  // average row of 256+ entries
+    const uint32_t G = longrows ? 64 : 16;
+    const uint32_t blocks = std::min<uint32_t>(std::max<uint32_t>(ceil_div((uint64_t)n * G, 256), 1), 4096);
+    if (dev_args) {
+        if (longrows)
+            k_pair_hash<true, 64><<<blocks, 256, 0, s>>>(JoinArgs{}, dev_args);
+        else
+            k_pair_hash<true, 16><<<blocks, 256, 0, s>>>(JoinArgs{}, dev_args);
+    } else {
+        if (longrows)
+            k_pair_hash<false, 64><<<blocks, 256, 0, s>>>(a, nullptr);
+        else
+            k_pair_hash<false, 16><<<blocks, 256, 0, s>>>(a, nullptr);
+    }
+    HIP_TRY(hipGetLastError());
+    return HEGJ_OK;
 }
 
 __global__ void k_zero_words(uint32_t *__restrict__ p, uint32_t n) {
