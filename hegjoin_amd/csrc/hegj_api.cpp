@@ -51,6 +51,88 @@ Scratch::~Scratch() {
     for (void *p : ptrs) (void)hipFree(p);
 }
 
+namespace {
+std::mutex g_cache_mu;
+std::map<void *, std::pair<int, size_t>> g_cache_live;           // ptr -> (device, size)
+std::multimap<std::pair<int, size_t>, void *> g_cache_free;      // (device, size) -> ptr
+size_t g_cache_bytes = 0;
+constexpr size_t kCacheCap = 16ull << 30;
+
+size_t cache_round(size_t b) {
+    if (b <= ((size_t)1 << 20)) {
+        size_t s = 256;
+        while (s < b) s <<= 1;
+        return s;
+    }
+    const size_t g = (size_t)2 << 20;
+    return (b + g - 1) / g * g;
+}
+
+void cache_release(int dev) {
+    std::vector<void *> rel;
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        for (auto it = g_cache_free.begin(); it != g_cache_free.end();) {
+            if (it->first.first == dev) {
+                rel.push_back(it->second);
+                g_cache_bytes -= it->first.second;
+                it = g_cache_free.erase(it);
+            } else {
+                ++it;
+            }
+        }
+    }
+    for (void *p : rel) (void)hipFree(p);
+}
+}  // namespace
+
+hipError_t dev_malloc(void **p, size_t bytes) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    const size_t rs = cache_round(bytes);
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        auto it = g_cache_free.find({dev, rs});
+        if (it != g_cache_free.end()) {
+            *p = it->second;
+            g_cache_free.erase(it);
+            g_cache_bytes -= rs;
+            g_cache_live[*p] = {dev, rs};
+            return hipSuccess;
+        }
+    }
+    e = hipMalloc(p, rs);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        cache_release(dev);
+        e = hipMalloc(p, rs);
+    }
+    if (e == hipSuccess) {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        g_cache_live[*p] = {dev, rs};
+    }
+    return e;
+}
+
+void dev_free(void *p) {
+    if (!p) return;
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        auto it = g_cache_live.find(p);
+        if (it != g_cache_live.end()) {
+            const auto key = it->second;
+            g_cache_live.erase(it);
+            if (g_cache_bytes + key.second <= kCacheCap) {
+                g_cache_free.insert({key, p});
+                g_cache_bytes += key.second;
+                return;
+            }
+        }
+    }
+    (void)hipFree(p);
+}
+
 // S7: largest s with RN(sqrt(s)) <= eps.  sqrt is monotone, so
 // sqrt(acc) <= eps  <=>  acc <= T, and the device needs no sqrt.
 double sqrt_threshold(double eps) {
@@ -127,10 +209,18 @@ struct hegj_result {
     uint64_t *d_row_off = nullptr;
     uint32_t *d_row_cnt = nullptr;
     hegj_result_info info{};
+    hipEvent_t done = nullptr;  // recorded when the join handed the result out
     ~hegj_result() {
-        for (uint32_t *b : batches) (void)hipFree(b);
-        if (d_row_off) (void)hipFree(d_row_off);
-        if (d_row_cnt) (void)hipFree(d_row_cnt);
+        // the buffers go back to the cache: nothing may still be writing them
+        if (done) {
+            (void)hipEventSynchronize(done);
+            (void)hipEventDestroy(done);
+        } else if (idx && idx->stream) {
+            (void)hipStreamSynchronize(idx->stream);  // failed inside the join
+        }
+        for (uint32_t *b : batches) dev_free(b);
+        dev_free(d_row_off);
+        dev_free(d_row_cnt);
     }
 };
 
@@ -286,13 +376,15 @@ struct JoinPlan {
     double estimate_ms = 0;
     bool f32 = false;  // FP32 filter with FP64 band recheck
     float tacc32 = 0, trej32 = 0;
+    hipStream_t stream = nullptr;  // the stream its kernels run on
     JoinPlan() = default;
     JoinPlan(const JoinPlan &) = delete;
     JoinPlan &operator=(const JoinPlan &) = delete;
     ~JoinPlan() {
+        if (stream) (void)hipStreamSynchronize(stream);
         for (void *p : {(void *)units, (void *)units_next, (void *)deferred, (void *)d_ctr,
                         (void *)d_ctr2, (void *)gmask})
-            if (p) (void)hipFree(p);
+            dev_free(p);
     }
 };
 
@@ -357,15 +449,16 @@ hegj_status make_plan(hegj_index *h, const hegj_join_opts &opts, uint32_t qb, co
                         "and needs a band below 1% of eps^2");
         P.f32 = ok;
     }
+    P.stream = s;
     HEGJ_TRY(build_units(h->g, h->ix, qb, order, &P.units, &P.U, s));
     const uint32_t U = P.U;
-    HIP_TRY(hipMalloc(&P.units_next, sizeof(uint2) * (U + 1)));
-    HIP_TRY(hipMalloc(&P.deferred, sizeof(uint32_t) * (U + 1)));
-    HIP_TRY(hipMalloc(&P.d_ctr, sizeof(Counters)));
-    HIP_TRY(hipMalloc(&P.d_ctr2, sizeof(Counters)));
+    HIP_TRY(dev_malloc_n(&P.units_next, (size_t)U + 1));
+    HIP_TRY(dev_malloc_n(&P.deferred, (size_t)U + 1));
+    HIP_TRY(dev_malloc_n(&P.d_ctr, 1));
+    HIP_TRY(dev_malloc_n(&P.d_ctr2, 1));
     HEGJ_TRY(join_occupancy(h->g.dim, qb, h->device, &P.max_blocks));
     // per-wave hit-mask scratch for units whose masks exceed LDS
-    HIP_TRY(hipMalloc(&P.gmask, sizeof(uint64_t) * (size_t)P.max_blocks * 4 * P.gmask_slots));
+    HIP_TRY(dev_malloc_n(&P.gmask, (size_t)P.max_blocks * 4 * P.gmask_slots));
     // unit ranges per XCD, split by estimated work
     P.parts = opts.xcd_parts ? std::min<uint32_t>(opts.xcd_parts, kMaxParts) : kMaxParts;
     if (U < P.parts * 256) P.parts = 1;
@@ -413,8 +506,8 @@ hegj_status run_batches(hegj_index *h, const hegj_join_opts &opts, uint32_t qb, 
     const uint32_t n = h->g.n, U = P.U;
     r->info.n_units = U;
     r->info.estimate_ms = P.estimate_ms;
-    HIP_TRY(hipMalloc(&r->d_row_off, sizeof(uint64_t) * n));
-    HIP_TRY(hipMalloc(&r->d_row_cnt, sizeof(uint32_t) * n));
+    HIP_TRY(dev_malloc_n(&r->d_row_off, n));
+    HIP_TRY(dev_malloc_n(&r->d_row_cnt, n));
     HIP_TRY(hipMemsetAsync(r->d_row_off, 0, sizeof(uint64_t) * n, s));
     HIP_TRY(hipMemsetAsync(r->d_row_cnt, 0, sizeof(uint32_t) * n, s));
     JoinArgs a = base_args(h, opts, P);
@@ -429,7 +522,7 @@ hegj_status run_batches(hegj_index *h, const hegj_join_opts &opts, uint32_t qb, 
     Counters c{};
     while (cur_n > 0) {
         uint32_t *buf = nullptr;
-        HIP_TRY(hipMalloc(&buf, sizeof(uint32_t) * capacity));
+        HIP_TRY(dev_malloc_n(&buf, capacity));
         r->batches.push_back(buf);
         r->bases.push_back(base);
         // reset per-batch counters (hash / evals / pairs accumulate)
@@ -461,7 +554,7 @@ hegj_status run_batches(hegj_index *h, const hegj_join_opts &opts, uint32_t qb, 
         base += capacity;
         if (c.n_deferred == cur_n) {
             // nothing fit: one unit alone exceeds the batch; grow it
-            (void)hipFree(r->batches.back());
+            dev_free(r->batches.back());
             r->batches.pop_back();
             r->bases.pop_back();
             base -= capacity;
@@ -591,12 +684,12 @@ hegj_status selfjoin_graph(hegj_index *h, const hegj_join_opts &opts, uint32_t q
     GraphPlan &G = *h->gplan;
     r->info.n_units = G.P.U;
     r->info.estimate_ms = G.P.estimate_ms;
-    HIP_TRY(hipMalloc(&r->d_row_off, sizeof(uint64_t) * n));
-    HIP_TRY(hipMalloc(&r->d_row_cnt, sizeof(uint32_t) * n));
+    HIP_TRY(dev_malloc_n(&r->d_row_off, n));
+    HIP_TRY(dev_malloc_n(&r->d_row_cnt, n));
     HIP_TRY(hipMemsetAsync(r->d_row_off, 0, sizeof(uint64_t) * n, s));
     HIP_TRY(hipMemsetAsync(r->d_row_cnt, 0, sizeof(uint32_t) * n, s));
     uint32_t *buf = nullptr;
-    HIP_TRY(hipMalloc(&buf, sizeof(uint32_t) * G.P.capacity));
+    HIP_TRY(dev_malloc_n(&buf, G.P.capacity));
     r->batches.push_back(buf);
     r->bases.push_back(0);
     JoinArgs a = base_args(h, opts, G.P);
@@ -657,11 +750,18 @@ hegj_status selfjoin_mfma(hegj_index *h, const uint32_t *order, hegj_result *r) 
     Scratch sc;
     uint2 *units = nullptr;
     uint32_t U = 0;
+    struct UnitsGuard {
+        hipStream_t s;
+        uint2 *&p;
+        ~UnitsGuard() {
+            (void)hipStreamSynchronize(s);
+            dev_free(p);
+        }
+    } units_guard{s, units};
     HEGJ_TRY(build_units(h->g, h->ix, gb_unit_queries(h->gb_kp / 32), order, &units, &U, s));
-    sc.ptrs.push_back(units);
     r->info.n_units = U;
-    HIP_TRY(hipMalloc(&r->d_row_off, sizeof(uint64_t) * n));
-    HIP_TRY(hipMalloc(&r->d_row_cnt, sizeof(uint32_t) * n));
+    HIP_TRY(dev_malloc_n(&r->d_row_off, n));
+    HIP_TRY(dev_malloc_n(&r->d_row_cnt, n));
     HIP_TRY(hipMemsetAsync(r->d_row_off, 0, sizeof(uint64_t) * n, s));
     HIP_TRY(hipMemsetAsync(r->d_row_cnt, 0, sizeof(uint32_t) * n, s));
     unsigned long long *d_hash;
@@ -939,6 +1039,12 @@ hegj_status hegj_tuning_load(const char *path, uint32_t *n_loaded) {
     return st;
 }
 
+static hegj_status mark_done(hegj_result *r, hipStream_t s) {
+    HIP_TRY(hipEventCreateWithFlags(&r->done, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(r->done, s));
+    return HEGJ_OK;
+}
+
 hegj_status hegj_selfjoin(hegj_index *h, const hegj_join_opts *opts_in, hegj_result **out) {
     if (!h || !out) return fail(HEGJ_ERR_INVALID_ARG, "NULL argument");
     auto t_start = std::chrono::steady_clock::now();
@@ -979,6 +1085,7 @@ hegj_status hegj_selfjoin(hegj_index *h, const hegj_join_opts *opts_in, hegj_res
         HEGJ_TRY(selfjoin_mfma(h, order, r.get()));
         r->info.total_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+        HEGJ_TRY(mark_done(r.get(), s));
         *out = r.release();
         return HEGJ_OK;
     }
@@ -1004,6 +1111,7 @@ hegj_status hegj_selfjoin(hegj_index *h, const hegj_join_opts *opts_in, hegj_res
     }
     r->info.total_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    HEGJ_TRY(mark_done(r.get(), s));
     *out = r.release();
     return HEGJ_OK;
 }

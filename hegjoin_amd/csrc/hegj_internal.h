@@ -93,6 +93,20 @@ struct Scratch {
     ~Scratch();
 };
 
+// Device buffers of the join (plan arrays, row tables, result batches) come
+// from a cache: a freed buffer is kept for the next join of the same shape
+// instead of going back through hipFree / hipMalloc, each of which costs
+// ~0.1 ms and synchronises the device.  dev_free only when the work using the
+// buffer has completed.  At most 16 GiB of freed buffers are kept; an
+// allocation that fails first releases them.  Pointers not from dev_malloc
+// are passed to hipFree.
+hipError_t dev_malloc(void **p, size_t bytes);
+template <typename T>
+hipError_t dev_malloc_n(T **p, size_t count) {
+    return dev_malloc(reinterpret_cast<void **>(p), count * sizeof(T));
+}
+void dev_free(void *p);
+
 // Largest s with RN(sqrt(s)) <= eps (the distance predicate without sqrt).
 double sqrt_threshold(double eps);
 

@@ -911,7 +911,7 @@ hegj_status build_units(const GridGeom &g, const DevIndex &ix, uint32_t qb, cons
     uint32_t U;
     HIP_TRY(hipMemcpyAsync(&U, d_total, sizeof(U), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    HIP_TRY(hipMalloc(units, sizeof(uint2) * (U + 1)));
+    HIP_TRY(dev_malloc_n(units, (size_t)U + 1));
     k_unit_fill<<<ceil_div(C, 256), 256, 0, s>>>(ix, qb, order, ucount, uoff, *units);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s));
