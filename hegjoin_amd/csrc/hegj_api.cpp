@@ -796,9 +796,11 @@ namespace hegj {
 const GridGeom &index_geom(const hegj_index *h) { return h->g; }
 const DevIndex &index_dev(const hegj_index *h) { return h->ix; }
 hegj_status select_device(int device) { return set_device(device); }
-hegj_status index_f32(hegj_index *h, const float **soa32, double *radius, const double **centre) {
+hegj_status index_f32(hegj_index *h, const float **soa32, const float **aos32, double *radius,
+                      const double **centre) {
     HEGJ_TRY(ensure_soa32(h));
     *soa32 = h->soa32;
+    *aos32 = h->aos32;
     *radius = h->f32_radius;
     *centre = h->f32_centre;
     return HEGJ_OK;

@@ -184,7 +184,8 @@ hegj_status launch_pair_hash(const JoinArgs &a, uint64_t pairs_hint, hipStream_t
 hegj_status make_soa32(const GridGeom &g, const DevIndex &ix, float **soa32, float **aos32, double *radius,
                        double **centre_dev, hipStream_t s);
 // The index's FP32 copy (built on first use), its radius and device centre.
-hegj_status index_f32(hegj_index *h, const float **soa32, double *radius, const double **centre);
+hegj_status index_f32(hegj_index *h, const float **soa32, const float **aos32, double *radius,
+                      const double **centre);
 // FP32-filter thresholds for threshold T (false: band wider than 1% of T).
 bool f32_thresholds(double T, uint32_t dim, double R, float *tacc, float *trej);
 // Whether the FP32 filter is instantiated for (dim, qb).
@@ -242,6 +243,7 @@ struct QJoinArgs {
     // FP32 filter (f32 != 0): centred float candidates and queries
     uint32_t f32;
     const float *soa32;
+    const float *aos32;          // point-major copy (dim <= kAosMaxDim)
     const float *q32;            // [nq * dim] row-major, centred like soa32
     float tacc32, trej32;
     unsigned long long *band;    // pairs rechecked in FP64 (fill pass)
@@ -259,6 +261,9 @@ hegj_status select_device(int device);
 hegj_status query_plan(QueryPlan *p, Scratch &keep, const QueryGeom &qg, const double *queries,
                        uint32_t nq, hipStream_t s);
 hegj_status query_run(const QJoinArgs &a, bool fill, int device, hipStream_t s);
+// *hash += sum of mix64(query << 32 | neighbour) over the CSR rows
+hegj_status query_hash(const uint64_t *offsets, const uint32_t *nbrs, uint32_t nq, uint64_t total,
+                       unsigned long long *hash, hipStream_t s);
 // counts[nq] -> offsets[nq + 1] (offsets[nq] = total)
 hegj_status query_offsets(const uint32_t *counts, uint32_t nq, uint64_t *offsets, hipStream_t s);
 

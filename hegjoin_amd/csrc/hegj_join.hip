@@ -28,6 +28,8 @@
 //      next batch (exact, no estimate can overflow a buffer).
 // The per-query (offset, count) table is the reference's neighborTable
 // (GPU.cu:1731-1776) with a single data array per batch.
+#include <type_traits>
+
 #include "hegj_internal.h"
 
 namespace hegj {
@@ -598,75 +600,92 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())
             // chunk only.  Each (query, chunk) ballot goes straight into lane
             // qi of `mine` (one writelane per half), so no mask stays live in
             // scalar registers across queries.
-            for (uint32_t ch0 = 0; ch0 < nchunks; ch0 += kDepth) {
+            // The loop is built for QL queries: QB, or fewer when the unit
+            // has fewer (cells of one or two points are common), so no FP
+            // work is spent on padding queries.  QL only bounds the loops;
+            // the query registers and masks keep their QB layout.
+            auto chunk_loop = [&](auto qlc) __attribute__((always_inline)) {
+                constexpr int QL = decltype(qlc)::value;
+                for (uint32_t ch0 = 0; ch0 < nchunks; ch0 += kDepth) {
 #pragma unroll
-                for (int p = 0; p < kDepth; p++) {
-                    const uint32_t ch = ch0 + p;
-                    if (!kRegs && ch >= nchunks) break;
-                    const bool valid = ch * 64 + lane < M;
-                    // single compares feed the ballots; validity is applied
-                    // to the scalar masks
-                    const uint64_t vmask = __builtin_amdgcn_ballot_w64(valid);
-                    if constexpr (F32) {
-                        float accf[QB];
+                    for (int p = 0; p < kDepth; p++) {
+                        const uint32_t ch = ch0 + p;
+                        if (!kRegs && ch >= nchunks) break;
+                        const bool valid = ch * 64 + lane < M;
+                        // single compares feed the ballots; validity is applied
+                        // to the scalar masks
+                        const uint64_t vmask = __builtin_amdgcn_ballot_w64(valid);
+                        if constexpr (F32) {
+                            float accf[QL];
 #pragma unroll
-                        for (int qi = 0; qi < QB; qi++) {
-                            accf[qi] = 0.0f;
+                            for (int qi = 0; qi < QL; qi++) {
+                                accf[qi] = 0.0f;
 #pragma unroll
-                            for (int l = 0; l < DR; l++) {
-                                const float dd = ringf[p][l] - qregf[qi][l];
-                                accf[qi] = __builtin_fmaf(dd, dd, accf[qi]);
+                                for (int l = 0; l < DR; l++) {
+                                    const float dd = ringf[p][l] - qregf[qi][l];
+                                    accf[qi] = __builtin_fmaf(dd, dd, accf[qi]);
+                                }
                             }
-                        }
-                        {
-                            const uint32_t tn = (ch + kDepth) * 64 + lane;
-                            load_cand32<DR>(a, tn < M ? cand_pos_adv(rl, nslots, tn, slot) : 0u, ringf[p]);
-                        }
+                            {
+                                const uint32_t tn = (ch + kDepth) * 64 + lane;
+                                load_cand32<DR>(a, tn < M ? cand_pos_adv(rl, nslots, tn, slot) : 0u, ringf[p]);
+                            }
 #pragma unroll
-                        for (int qi = 0; qi < QB; qi++) {
-                            const uint64_t vq = (uint32_t)qi < nq ? vmask : 0ull;
-                            const uint64_t m = __builtin_amdgcn_ballot_w64(accf[qi] <= a.tacc32) & vq;
-                            const uint64_t mb = vq & ~m & ~__builtin_amdgcn_ballot_w64(accf[qi] > a.trej32);
-                            if (mb) {
-                                // decided after the loop by the exact FP64 chain
-                                if (lane == 0 && nband < (uint32_t)kBandList) rl.band[nband] = (ch << 4) | (uint32_t)qi;
-                                nband++;
-                                my_band += (uint64_t)__popcll(mb);
+                            for (int qi = 0; qi < QL; qi++) {
+                                const uint64_t vq = (uint32_t)qi < nq ? vmask : 0ull;
+                                const uint64_t m = __builtin_amdgcn_ballot_w64(accf[qi] <= a.tacc32) & vq;
+                                const uint64_t mb = vq & ~m & ~__builtin_amdgcn_ballot_w64(accf[qi] > a.trej32);
+                                if (mb) {
+                                    // decided after the loop by the exact FP64 chain
+                                    if (lane == 0 && nband < (uint32_t)kBandList) rl.band[nband] = (ch << 4) | (uint32_t)qi;
+                                    nband++;
+                                    my_band += (uint64_t)__popcll(mb);
+                                }
+                                mine = set_lane_u64(mine, m, qi);
                             }
-                            mine = set_lane_u64(mine, m, qi);
-                        }
-                    } else {
-                        double acc[QB];
-                        if constexpr (kRegs) {
-                            // use ring[p], then refill it with chunk ch + kDepth
-                            // (no copy of registers that still have loads in flight)
-                            if constexpr (kQS) {
-                                dist_regs<DR, QB>(ring[p], qreg, acc);
-                            } else {
-                                // opaque offset: keeps the compiler from hoisting
-                                // QB * D loop-invariant LDS reads into registers
-                                uint32_t qoff = (uint32_t)wid * QB * DR;
-                                __asm__ volatile("" : "+v"(qoff));
-                                dist_regs_lds<DR, QB>(ring[p], q_dyn + qoff, acc);
-                            }
-                            const uint32_t tn = (ch + kDepth) * 64 + lane;
-                            const bool vn = tn < M;
-                            load_cand<DR>(a, vn ? cand_pos_adv(rl, nslots, tn, slot) : 0u, vn, ring[p]);
                         } else {
-                            const uint32_t t = ch * 64 + lane;
-                            const uint32_t pos = valid ? cand_pos_adv(rl, nslots, t, slot) : 0u;
-                            uint32_t qoff = (uint32_t)wid * QB * dimq;
-                            __asm__ volatile("" : "+v"(qoff));
-                            dist_stream<D, QB>(a, pos, valid, q_dyn + qoff, nq, acc);
-                        }
+                            double acc[QL];
+                            if constexpr (kRegs) {
+                                // use ring[p], then refill it with chunk ch + kDepth
+                                // (no copy of registers that still have loads in flight)
+                                if constexpr (kQS) {
+                                    dist_regs<DR, QL>(ring[p], qreg, acc);
+                                } else {
+                                    // opaque offset: keeps the compiler from hoisting
+                                    // QB * D loop-invariant LDS reads into registers
+                                    uint32_t qoff = (uint32_t)wid * QB * DR;
+                                    __asm__ volatile("" : "+v"(qoff));
+                                    dist_regs_lds<DR, QL>(ring[p], q_dyn + qoff, acc);
+                                }
+                                const uint32_t tn = (ch + kDepth) * 64 + lane;
+                                const bool vn = tn < M;
+                                load_cand<DR>(a, vn ? cand_pos_adv(rl, nslots, tn, slot) : 0u, vn, ring[p]);
+                            } else {
+                                const uint32_t t = ch * 64 + lane;
+                                const uint32_t pos = valid ? cand_pos_adv(rl, nslots, t, slot) : 0u;
+                                uint32_t qoff = (uint32_t)wid * QB * dimq;
+                                __asm__ volatile("" : "+v"(qoff));
+                                dist_stream<D, QB>(a, pos, valid, q_dyn + qoff, nq, acc);
+                            }
 #pragma unroll
-                        for (int qi = 0; qi < QB; qi++)
-                            mine = set_lane_u64(mine, __builtin_amdgcn_ballot_w64(acc[qi] <= T) &
-                                                          ((uint32_t)qi < nq ? vmask : 0ull), qi);
+                            for (int qi = 0; qi < QL; qi++)
+                                mine = set_lane_u64(mine, __builtin_amdgcn_ballot_w64(acc[qi] <= T) &
+                                                              ((uint32_t)qi < nq ? vmask : 0ull), qi);
+                        }
+                        cnt_l += (uint32_t)__popcll(mine);
+                        if (keep && (uint32_t)lane < nq && ch < nchunks) hs[ch * nq + lane] = mine;
                     }
-                    cnt_l += (uint32_t)__popcll(mine);
-                    if (keep && (uint32_t)lane < nq && ch < nchunks) hs[ch * nq + lane] = mine;
                 }
+            };
+            if constexpr (kRegs && QB >= 4) {
+                if (nq == 1)
+                    chunk_loop(std::integral_constant<int, 1>{});
+                else if (nq <= (uint32_t)QB / 2)
+                    chunk_loop(std::integral_constant<int, QB / 2>{});
+                else
+                    chunk_loop(std::integral_constant<int, QB>{});
+            } else {
+                chunk_loop(std::integral_constant<int, QB>{});
             }
             if constexpr (F32) {
                 // The band: the reference's exact FP64 chain decides the lanes

@@ -77,6 +77,26 @@ __global__ __launch_bounds__(256) void k_qrun_fill(const uint32_t *__restrict__ 
     if (i < nq && flag[i]) starts[rank[i]] = i;
 }
 
+// Units of at most kUnitQueries queries: a run of queries sharing a cell is
+// split so that a dense cell spreads over many waves instead of one.
+constexpr uint32_t kUnitQueries = 16;
+
+__global__ __launch_bounds__(256) void k_sub_counts(const uint32_t *__restrict__ starts, uint32_t nr,
+                                                    uint32_t *__restrict__ cnt) {
+    const uint32_t r = blockIdx.x * 256 + threadIdx.x;
+    if (r < nr) cnt[r] = (starts[r + 1] - starts[r] + kUnitQueries - 1) / kUnitQueries;
+}
+
+__global__ __launch_bounds__(256) void k_sub_fill(const uint32_t *__restrict__ starts, uint32_t nr,
+                                                  const uint32_t *__restrict__ first,
+                                                  uint32_t *__restrict__ sub) {
+    const uint32_t r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= nr) return;
+    const uint32_t b = starts[r], e = starts[r + 1];
+    uint32_t o = first[r];
+    for (uint32_t i = b; i < e; i += kUnitQueries) sub[o++] = i;
+}
+
 // Stencil rows of the unit whose widened key is `key` into the wave's LDS
 // list; returns the candidate count.
 __device__ uint32_t qbuild_rows(const QJoinArgs &a, uint64_t key, QRowList &rl, uint32_t *nslots) {
@@ -148,7 +168,7 @@ __global__ __launch_bounds__(64 * kQWaves) void k_qjoin(QJoinArgs a) {
     const uint64_t n = a.qg.g.n;
     const double T = a.thresh;
     const double *__restrict__ soa = a.ix.soa;
-    uint64_t my_hash = 0, my_evals = 0, my_band = 0;
+    uint64_t my_evals = 0, my_band = 0;
     while (true) {
         uint32_t u = 0;
         if (lane == 0) u = atomicAdd(a.queue_head, 1u);
@@ -165,7 +185,7 @@ __global__ __launch_bounds__(64 * kQWaves) void k_qjoin(QJoinArgs a) {
             uint32_t cnt[kQB];
 #pragma unroll
             for (int qi = 0; qi < kQB; qi++) {
-                qid[qi] = a.ids[g0 + min((uint32_t)qi, nqg - 1)];
+                qid[qi] = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.ids[g0 + min((uint32_t)qi, nqg - 1)]);
                 base[qi] = FILL ? a.offsets[qid[qi]] : 0;
                 cnt[qi] = 0;
             }
@@ -214,15 +234,29 @@ __global__ __launch_bounds__(64 * kQWaves) void k_qjoin(QJoinArgs a) {
                     while (slot + 1 < nslots && rl.off[slot + 1] <= t) slot++;
                     pos = rl.start[slot] + (t - rl.off[slot]);
                 }
+                // one compare per ballot, validity applied to the scalar
+                // masks; the hit lanes of a fill store under the mask itself
+                const uint64_t vmask = __builtin_amdgcn_ballot_w64(valid);
+                uint32_t pid = 0;
+                if (FILL) pid = a.ix.ila[valid ? pos : 0u];
                 double acc[kQB];
 #pragma unroll
                 for (int qi = 0; qi < kQB; qi++) acc[qi] = 0.0;
                 if constexpr (F32) {
                     float x[DQ];
+                    if constexpr (DQ % 2 == 0) {
+                        typedef float f2 __attribute__((ext_vector_type(2)));
+                        const f2 *src = reinterpret_cast<const f2 *>(a.aos32 + (uint64_t)pos * DQ);
 #pragma unroll
-                    for (int l = 0; l < DQ; l++) x[l] = a.soa32[(uint64_t)l * n + pos];
-                    uint32_t pid = 0;
-                    if (FILL && valid) pid = a.ix.ila[pos];
+                        for (int l = 0; l < DQ / 2; l++) {
+                            const f2 v = src[l];
+                            x[2 * l] = v.x;
+                            x[2 * l + 1] = v.y;
+                        }
+                    } else {
+#pragma unroll
+                        for (int l = 0; l < DQ; l++) x[l] = a.soa32[(uint64_t)l * n + pos];
+                    }
 #pragma unroll
                     for (int qi = 0; qi < kQB; qi++) {
                         float accf = 0.0f;
@@ -231,32 +265,40 @@ __global__ __launch_bounds__(64 * kQWaves) void k_qjoin(QJoinArgs a) {
                             const float d = x[l] - qregf[qi][l];
                             accf = __builtin_fmaf(d, d, accf);
                         }
-                        const bool live_q = valid && (uint32_t)qi < nqg;
-                        bool hit = live_q && accf <= a.tacc32;
-                        const bool band = live_q && !(accf <= a.tacc32) && !(accf > a.trej32);
-                        if (__ballot(band)) {
-                            double e = 0.0;
+                        const uint64_t vq = (uint32_t)qi < nqg ? vmask : 0ull;
+                        uint64_t m = __builtin_amdgcn_ballot_w64(accf <= a.tacc32) & vq;
+                        const uint64_t mb = vq & ~m & ~__builtin_amdgcn_ballot_w64(accf > a.trej32);
+                        if (mb) {
+                            // the band: the exact FP64 chain decides
+                            bool hit = false;
+                            if (__builtin_amdgcn_inverse_ballot_w64(mb)) {
+                                double e = 0.0;
 #pragma unroll
-                            for (int l = 0; l < DQ; l++) {
-                                const double d = soa[(uint64_t)l * n + pos] - a.queries[(uint64_t)qid[qi] * dim + l];
-                                e = __builtin_fma(d, d, e);
+                                for (int l = 0; l < DQ; l++) {
+                                    const double d = soa[(uint64_t)l * n + pos] - a.queries[(uint64_t)qid[qi] * dim + l];
+                                    e = __builtin_fma(d, d, e);
+                                }
+                                hit = e <= T;
                             }
-                            hit |= band && e <= T;
-                            my_band += (uint64_t)__popcll(__ballot(band));
+                            m |= __builtin_amdgcn_ballot_w64(hit);
+                            my_band += (uint64_t)__popcll(mb);
                         }
-                        const uint64_t m = __ballot(hit);
-                        if (FILL && hit) {
+                        if (FILL && m && __builtin_amdgcn_inverse_ballot_w64(m))
                             a.out[base[qi] + cnt[qi] + lanes_below(m)] = pid;
-                            my_hash += mix64(((uint64_t)qid[qi] << 32) | pid);
-                        }
                         cnt[qi] += (uint32_t)__popcll(m);
                     }
                     continue;
                 } else if constexpr (D > 0) {
                     double x[DQ];
                     // unconditional (invalid lanes read position 0, masked below)
+                    if constexpr (D <= (int)kAosMaxDim) {
+                        const double *src = a.ix.aos + (uint64_t)pos * DQ;
 #pragma unroll
-                    for (int l = 0; l < DQ; l++) x[l] = soa[(uint64_t)l * n + pos];
+                        for (int l = 0; l < DQ; l++) x[l] = src[l];
+                    } else {
+#pragma unroll
+                        for (int l = 0; l < DQ; l++) x[l] = soa[(uint64_t)l * n + pos];
+                    }
 #pragma unroll
                     for (int qi = 0; qi < kQB; qi++) {
 #pragma unroll
@@ -275,16 +317,11 @@ __global__ __launch_bounds__(64 * kQWaves) void k_qjoin(QJoinArgs a) {
                         }
                     }
                 }
-                uint32_t pid = 0;
-                if (FILL && valid) pid = a.ix.ila[pos];
 #pragma unroll
                 for (int qi = 0; qi < kQB; qi++) {
-                    const bool hit = valid && (uint32_t)qi < nqg && acc[qi] <= T;
-                    const uint64_t m = __ballot(hit);
-                    if (FILL && hit) {
+                    const uint64_t m = __builtin_amdgcn_ballot_w64(acc[qi] <= T) & ((uint32_t)qi < nqg ? vmask : 0ull);
+                    if (FILL && m && __builtin_amdgcn_inverse_ballot_w64(m))
                         a.out[base[qi] + cnt[qi] + lanes_below(m)] = pid;
-                        my_hash += mix64(((uint64_t)qid[qi] << 32) | pid);
-                    }
                     cnt[qi] += (uint32_t)__popcll(m);
                 }
             }
@@ -299,13 +336,27 @@ __global__ __launch_bounds__(64 * kQWaves) void k_qjoin(QJoinArgs a) {
         }
         wave_lds_sync();  // the row list is rebuilt for the next unit
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) my_hash += __shfl_xor(my_hash, o, 64);
     if (lane == 0) {
-        if (FILL && my_hash) atomicAdd(a.hash, (unsigned long long)my_hash);
         if (!FILL && my_evals) atomicAdd(a.evals, (unsigned long long)my_evals);
         if (FILL && my_band && a.band) atomicAdd(a.band, (unsigned long long)my_band);
     }
+}
+
+// Pair hash of a request's CSR result: sum of mix64(query << 32 | neighbour)
+// over its rows, G lanes per row; one atomic per wave.
+template <int G>
+__global__ __launch_bounds__(256) void k_csr_hash(const uint64_t *__restrict__ off, const uint32_t *__restrict__ nb,
+                                                  uint32_t n, unsigned long long *hash) {
+    const uint32_t sub = threadIdx.x & (G - 1);
+    const uint32_t grp = (blockIdx.x * 256 + threadIdx.x) / G, ngrp = (gridDim.x * 256) / G;
+    uint64_t h = 0;
+    for (uint32_t q = grp; q < n; q += ngrp) {
+        const uint64_t b = off[q], e = off[q + 1];
+        for (uint64_t j = b + sub; j < e; j += G) h += mix64(((uint64_t)q << 32) | nb[j]);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) h += __shfl_xor(h, o, 64);
+    if ((threadIdx.x & 63) == 0 && h) atomicAdd(hash, (unsigned long long)h);
 }
 
 // q32 = fl32(fl64(q - centre)) and the largest |q - centre| (as ordered bits).
@@ -400,10 +451,30 @@ hegj_status query_plan(QueryPlan *p, Scratch &keep, const QueryGeom &qg, const d
     uint32_t h[2];
     HIP_TRY(hipMemcpyAsync(h, d_small, sizeof(h), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    p->n_units = h[0];
+    const uint32_t runs = h[0];
     p->n_live = h[1];
-    // starts[n_units] = end of the live queries
-    HIP_TRY(hipMemcpyAsync(p->starts + p->n_units, &p->n_live, sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    // starts[runs] = end of the live queries
+    HIP_TRY(hipMemcpyAsync(p->starts + runs, &p->n_live, sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    // split the runs into units of at most kUnitQueries queries
+    p->n_units = 0;
+    if (runs > 0) {
+        uint32_t *cnt, *first, *sub_starts, *d_total;
+        HIP_TRY(sc.get(&cnt, runs));
+        HIP_TRY(sc.get(&first, runs));
+        HIP_TRY(sc.get(&sub_starts, (size_t)nq + 1));
+        HIP_TRY(sc.get(&d_total, 1));
+        k_sub_counts<<<ceil_div(runs, 256), 256, 0, s>>>(p->starts, runs, cnt);
+        HIP_TRY(hipGetLastError());
+        HEGJ_TRY(scan_u32(cnt, first, runs, d_total, s));
+        k_sub_fill<<<ceil_div(runs, 256), 256, 0, s>>>(p->starts, runs, first, sub_starts);
+        HIP_TRY(hipGetLastError());
+        uint32_t units = 0;
+        HIP_TRY(hipMemcpyAsync(&units, d_total, sizeof(units), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        HIP_TRY(hipMemcpyAsync(p->starts, sub_starts, sizeof(uint32_t) * units, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(p->starts + units, &p->n_live, sizeof(uint32_t), hipMemcpyHostToDevice, s));
+        p->n_units = units;
+    }
     HIP_TRY(hipStreamSynchronize(s));
     return HEGJ_OK;
 }
@@ -423,6 +494,20 @@ hegj_status query_run(const QJoinArgs &base, bool fill, int device, hipStream_t 
     const uint32_t blocks =
         std::max<uint32_t>(1, std::min<uint32_t>(resident, ceil_div(base.n_units, kQWaves)));
     return fill ? launch_qd<true>(base, blocks, s) : launch_qd<false>(base, blocks, s);
+}
+
+hegj_status query_hash(const uint64_t *offsets, const uint32_t *nbrs, uint32_t nq, uint64_t total,
+                       unsigned long long *hash, hipStream_t s) {
+    if (nq == 0 || total == 0) return HEGJ_OK;
+    const bool longrows = total / nq >= 256;
+    const uint32_t G = longrows ? 64 : 16;
+    const uint32_t blocks = std::min<uint32_t>(std::max<uint32_t>(ceil_div((uint64_t)nq * G, 256), 1), 4096);
+    if (longrows)
+        k_csr_hash<64><<<blocks, 256, 0, s>>>(offsets, nbrs, nq, hash);
+    else
+        k_csr_hash<16><<<blocks, 256, 0, s>>>(offsets, nbrs, nq, hash);
+    HIP_TRY(hipGetLastError());
+    return HEGJ_OK;
 }
 
 hegj_status query_to_f32(const double *q, uint32_t nq, uint32_t dim, const double *centre, float *q32,

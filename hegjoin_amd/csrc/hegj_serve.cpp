@@ -110,6 +110,7 @@ struct hegj_server {
     std::vector<QueryGeom> qgeo;
     struct LevelF32 {
         const float *soa32 = nullptr;
+        const float *aos32 = nullptr;
         const double *centre = nullptr;
         double radius = 0;
     };
@@ -216,6 +217,7 @@ hegj_status hegj_server::run(Request *r, hipStream_t s, Arena &arena, void *ctr_
     if (f32) {
         a.f32 = 1;
         a.soa32 = lf32[r->level].soa32;
+        a.aos32 = lf32[r->level].aos32;
         a.q32 = q32;
         a.tacc32 = tacc;
         a.trej32 = trej;
@@ -236,6 +238,7 @@ hegj_status hegj_server::run(Request *r, hipStream_t s, Arena &arena, void *ctr_
     a.offsets = res_off;
     a.out = res_nb;
     HEGJ_TRY(query_run(a, true, device, s));
+    HEGJ_TRY(query_hash(res_off, res_nb, nq, total, &ctr->hash, s));
     Ctr c;
     HIP_TRY(hipMemcpyAsync(&c, ctr, sizeof(c), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -371,7 +374,7 @@ hegj_status hegj_serve_open(const double *database, uint32_t n, const hegj_serve
         S->levels.push_back(h);
         if (o->dim == 2 || o->dim == 6) {
             hegj_server::LevelF32 L;
-            st = index_f32(h, &L.soa32, &L.radius, &L.centre);
+            st = index_f32(h, &L.soa32, &L.aos32, &L.radius, &L.centre);
             if (st != HEGJ_OK) {
                 (void)hipFree(dev);
                 return bail(st);
