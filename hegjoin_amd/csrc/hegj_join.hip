@@ -603,7 +603,9 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())
             // The loop is built for QL queries: QB, or fewer when the unit
             // has fewer (cells of one or two points are common), so no FP
             // work is spent on padding queries.  QL only bounds the loops;
-            // the query registers and masks keep their QB layout.
+            // the query registers and masks keep their QB layout.  (Only the
+            // small-D, QB <= 8 instances get the extra copies: each is a
+            // fully unrolled loop, and compile time grows with them.)
             auto chunk_loop = [&](auto qlc) __attribute__((always_inline)) {
                 constexpr int QL = decltype(qlc)::value;
                 for (uint32_t ch0 = 0; ch0 < nchunks; ch0 += kDepth) {
@@ -677,7 +679,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())
                     }
                 }
             };
-            if constexpr (kRegs && QB >= 4) {
+            if constexpr (kRegs && D <= 8 && QB >= 4 && QB <= 8) {
                 if (nq == 1)
                     chunk_loop(std::integral_constant<int, 1>{});
                 else if (nq <= (uint32_t)QB / 2)
@@ -787,59 +789,66 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())
             // (indexLookupArr) and, when kept, its masks (lane qi: query qi's)
             // -- are issued kW chunks ahead, so no iteration waits on a
             // fresh global load.
-            constexpr int kW = 8;
-            uint32_t idr[kW];
-            uint64_t mkr[kW];
-            uint32_t slot2 = 0, slot3 = 0;
-            auto fetch = [&](uint32_t c, uint32_t &id, uint64_t &mk) {
-                const uint32_t t = c * 64 + lane;
-                id = gld(a.ix.ila + (t < M ? cand_pos_adv(rl, nslots, t, slot2) : 0u));
-                mk = (keep && c < nchunks && (uint32_t)lane < nq) ? hs[c * nq + lane] : 0ull;
-            };
+            constexpr int kW = QB <= 8 ? 8 : 4;
+            uint32_t slot2 = 0;
+            if (keep) {
+                uint32_t idr[kW];
+                uint64_t mkr[kW];
+                auto fetch = [&](uint32_t c, uint32_t &id, uint64_t &mk) {
+                    const uint32_t t = c * 64 + lane;
+                    id = gld(a.ix.ila + (t < M ? cand_pos_adv(rl, nslots, t, slot2) : 0u));
+                    mk = (c < nchunks && (uint32_t)lane < nq) ? hs[c * nq + lane] : 0ull;
+                };
 #pragma unroll
-            for (int p = 0; p < kW; p++) fetch((uint32_t)p, idr[p], mkr[p]);
-            for (uint32_t ch0 = 0; ch0 < nchunks; ch0 += kW) {
+                for (int p = 0; p < kW; p++) fetch((uint32_t)p, idr[p], mkr[p]);
+                for (uint32_t ch0 = 0; ch0 < nchunks; ch0 += kW) {
 #pragma unroll
-                for (int p = 0; p < kW; p++) {
-                    const uint32_t ch = ch0 + p;
-                    if (ch >= nchunks) break;
-                    const uint32_t id = idr[p];
-                    const uint64_t mk = mkr[p];
-                    fetch(ch + kW, idr[p], mkr[p]);
+                    for (int p = 0; p < kW; p++) {
+                        const uint32_t ch = ch0 + p;
+                        if (ch >= nchunks) break;
+                        const uint32_t id = idr[p];
+                        const uint64_t mk = mkr[p];
+                        fetch(ch + kW, idr[p], mkr[p]);
+#pragma unroll
+                        for (int qi = 0; qi < QB; qi++) {
+                            // lanes >= nq of mk hold zero masks
+                            const uint64_t m =
+                                ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(mk >> 32), qi) << 32) |
+                                (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)mk, qi);
+                            if (m == 0) continue;
+                            if (__builtin_amdgcn_inverse_ballot_w64(m)) gst(a.out + oq[qi] + lanes_below(m), id);
+                            oq[qi] += (uint64_t)__popcll(m);
+                        }
+                    }
+                }
+            } else {
+                // masks did not fit anywhere (very large units): evaluate
+                // each chunk again, one query at a time
+                for (uint32_t ch = 0; ch < nchunks; ch++) {
                     const uint32_t t = ch * 64 + lane;
                     const bool valid = t < M;
-                    uint32_t pos = 0;
+                    const uint32_t pos = valid ? cand_pos_adv(rl, nslots, t, slot2) : 0u;
+                    const uint32_t id = gld(a.ix.ila + pos);
                     double x[DR];
-                    if (!keep) {
-                        // masks did not fit anywhere (very large units): evaluate
-                        // the chunk again, one query at a time
-                        pos = valid ? cand_pos_adv(rl, nslots, t, slot3) : 0u;
-                        if constexpr (kRegs) load_cand<DR>(a, pos, valid, x);
-                        my_evals += (uint64_t)min(64u, M - ch * 64) * nq;
-                    }
+                    if constexpr (kRegs) load_cand<DR>(a, pos, valid, x);
+                    my_evals += (uint64_t)min(64u, M - ch * 64) * nq;
 #pragma unroll
                     for (int qi = 0; qi < QB; qi++) {
-                        uint64_t m = 0;
-                        if ((uint32_t)qi >= nq) {
-                        } else if (keep) {
-                            m = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(mk >> 32), qi) << 32) |
-                                (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)mk, qi);
-                        } else {
-                            double acc1 = 0.0;
-                            if constexpr (kRegs) {
+                        if ((uint32_t)qi >= nq) continue;
+                        double acc1 = 0.0;
+                        if constexpr (kRegs) {
 #pragma unroll
-                                for (int l = 0; l < DR; l++) {
-                                    const double q = F32 ? a.ix.soa[(uint64_t)l * a.g.n + qpos0 + qi]
-                                                     : kQS ? qreg[kQS && !F32 ? qi : 0][kQS && !F32 ? l : 0]
-                                                           : qs[qi * DR + l];
-                                    const double dd = x[l] - q;
-                                    acc1 = __builtin_fma(dd, dd, acc1);
-                                }
-                            } else {
-                                dist_stream<D, 1>(a, pos, valid, qs + qi * dimq, 1, &acc1);
+                            for (int l = 0; l < DR; l++) {
+                                const double q = F32 ? a.ix.soa[(uint64_t)l * a.g.n + qpos0 + qi]
+                                                 : kQS ? qreg[kQS && !F32 ? qi : 0][kQS && !F32 ? l : 0]
+                                                       : qs[qi * DR + l];
+                                const double dd = x[l] - q;
+                                acc1 = __builtin_fma(dd, dd, acc1);
                             }
-                            m = __ballot(valid && acc1 <= T);
+                        } else {
+                            dist_stream<D, 1>(a, pos, valid, qs + qi * dimq, 1, &acc1);
                         }
+                        const uint64_t m = __ballot(valid && acc1 <= T);
                         if (m == 0) continue;
                         if (__builtin_amdgcn_inverse_ballot_w64(m)) gst(a.out + oq[qi] + lanes_below(m), id);
                         oq[qi] += (uint64_t)__popcll(m);
