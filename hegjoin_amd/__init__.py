@@ -123,6 +123,8 @@ _lib.hegj_dataset_load.argtypes = [C.c_char_p, C.c_uint32, C.POINTER(C.POINTER(C
                                    C.POINTER(C.c_uint32)]
 _lib.hegj_dataset_free.argtypes = [_vp]
 _lib.hegj_stream_copy_bandwidth.argtypes = [C.c_uint64, C.c_int, C.POINTER(C.c_double)]
+_lib.hegj_release_cached_memory.argtypes = [C.c_int]
+_lib.hegj_release_cached_memory.restype = None
 _lib.hegj_index_opts_default.argtypes = [C.POINTER(_IndexOpts), C.c_uint32]
 _lib.hegj_index_build.argtypes = [_vp, C.c_uint32, C.c_double, C.POINTER(_IndexOpts), C.POINTER(_vp)]
 _lib.hegj_index_build_device.argtypes = _lib.hegj_index_build.argtypes
@@ -191,6 +193,11 @@ def stream_copy_bandwidth(nbytes=1 << 30, device=0):
     g = C.c_double()
     _check(_lib.hegj_stream_copy_bandwidth(nbytes, device, C.byref(g)))
     return g.value
+
+
+def release_cached_memory(device=-1):
+    """Return the device buffers the self-join keeps for reuse (-1: all devices)."""
+    _lib.hegj_release_cached_memory(device)
 
 
 def library():

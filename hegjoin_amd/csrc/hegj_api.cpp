@@ -86,6 +86,8 @@ void cache_release(int dev) {
 }
 }  // namespace
 
+void cache_release_device(int dev) { cache_release(dev); }
+
 hipError_t dev_malloc(void **p, size_t bytes) {
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
@@ -832,6 +834,16 @@ int hegj_device_count(void) {
     int c = 0;
     if (hipGetDeviceCount(&c) != hipSuccess) return 0;
     return c;
+}
+
+void hegj_release_cached_memory(int device) {
+    if (device >= 0) {
+        hegj::cache_release_device(device);
+        return;
+    }
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return;
+    for (int d = 0; d < n; d++) hegj::cache_release_device(d);
 }
 
 hegj_status hegj_stream_copy_bandwidth(uint64_t bytes, int device, double *gbps) {

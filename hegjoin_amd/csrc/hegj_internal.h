@@ -106,6 +106,7 @@ hipError_t dev_malloc_n(T **p, size_t count) {
     return dev_malloc(reinterpret_cast<void **>(p), count * sizeof(T));
 }
 void dev_free(void *p);
+void cache_release_device(int dev);  // hipFree the cached buffers of one device
 
 // Largest s with RN(sqrt(s)) <= eps (the distance predicate without sqrt).
 double sqrt_threshold(double eps);

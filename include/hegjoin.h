@@ -81,6 +81,11 @@ int hegj_device_count(void);
  * of the reference; it is the measured roof the join's HBM figures are
  * compared with. */
 hegj_status hegj_stream_copy_bandwidth(uint64_t bytes, int device, double *gbps);
+/* The self-join keeps the device buffers of finished calls (work units, row
+ * tables, result batches) for reuse by later calls, up to 16 GiB per process
+ * (an allocation that fails releases them first).  This returns the unused
+ * ones on `device` (-1: every device).  No reference counterpart. */
+void hegj_release_cached_memory(int device);
 
 /* ------------------------------------------------------------------------ */
 /* Datasets.                                                                */

@@ -294,3 +294,9 @@ def test_f32_thresholds_reject_wide_bands():
     with pytest.raises(H.HegjError) as e:
         H.f32_thresholds(1e-6, 6, 1e3)  # tiny eps, huge coordinates
     assert e.value.status == 2
+
+
+def test_release_cached_memory_is_safe_without_a_device():
+    import hegjoin_amd as H
+    H.release_cached_memory()
+    H.release_cached_memory(0)

@@ -420,3 +420,15 @@ def test_f32_band_list_overflow(H, oracle, qb):
     eoff, enb = oracle.join(pts, 6, eps)
     assert info["refine_used"] == "f32" and info["band_rechecks"] > 64 * 64
     assert np.array_equal(off, eoff) and np.array_equal(nb, enb)
+
+
+def test_cached_buffers_released_between_joins(H, oracle):
+    """Joins reuse the buffers of finished calls; releasing them between
+    calls changes nothing in the results."""
+    pts = oracle.generate_expo(20000, 6, seed=91)
+    eoff, enb = oracle.join(pts, 6, 0.015)
+    for release in (False, True, False):
+        if release:
+            H.release_cached_memory()
+        off, nb, _ = gpu_join(H, pts, 0.015, 6)
+        assert np.array_equal(off, eoff) and np.array_equal(nb, enb)
