@@ -47,7 +47,11 @@ SERVE = dict(dim=6, n=10_000_000, k=6, nq_lo=1_000, nq_hi=500_000, eps_lo=0.002,
              name="serving: 6-D 10M expo FP64 resident index; Poisson arrivals; request queries "
                   "log-uniform 1k-500k, eps log-uniform [0.002, 0.02]")
 PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
-PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector with packed (2-wide) FP32 instructions
+# This build compiles without packed FP32 (v_pk_*_f32), so an FP32 VALU
+# instruction does one operation per lane: the roof it can reach is half the
+# packed peak.
+PEAK_FP32_UNPACKED_TFLOPS = PEAK_FP32_TFLOPS / 2
 PEAK_BF16_TFLOPS = 2500.0 # MI355X dense bf16 MFMA
 PEAK_HBM_GBS = 8000.0     # MI355X HBM3E
 
@@ -66,32 +70,53 @@ def calibrate(dim, n, target, ns=10000):
     return float(np.partition(d, m)[m])
 
 
+def host_cpu():
+    """CPU model, CPUs visible to this process, and the threads the baseline
+    uses: OMP_NUM_THREADS when set (a GPU box sets it to the host CPU share of
+    its one GPU), else every CPU the process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    visible = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = int(env) if env.isdigit() and int(env) > 0 else visible
+    return model, visible, min(threads, visible)
+
+
 def cpu_baseline(pts, eps, dim, seconds_budget=20.0):
     """The reference's Super-EGO on a seeded query sample (oracle/_ref), or the
     oracle's restatement when that binary is absent."""
     from oracle import pyoracle as O
     n = len(pts)
     order = np.random.RandomState(7).permutation(n).astype(np.uint32)
+    model, visible, threads = host_cpu()
     # probe 20000 queries (small probes overstate the per-query cost: thread
     # start-up) to size a sample of about seconds_budget of join time
     if O.ref_available(dim):
         probe_n = min(n, 20000)
-        probe = O.ref_superego_time(pts, eps, order, probe_n)
+        probe = O.ref_superego_time(pts, eps, order, probe_n, threads=threads)
         per_q = max(probe["join_s"], 1e-6) / probe_n
         nq = int(min(n, max(probe_n, seconds_budget / per_q)))
-        r = O.ref_superego_time(pts, eps, order, nq)
+        r = O.ref_superego_time(pts, eps, order, nq, threads=threads)
         return {"value": r["pairs"] / r["join_s"], "unit": "result pairs/s",
-                "cores": r["threads"], "kind": "reference",
+                "cores": r["threads"], "kind": "reference", "cpu_model": model,
+                "host_cpus_visible": visible,
                 "sample": f"Super-EGO (reference Util.cpp/MultiThreadJoin.cpp built from source) "
                           f"join of {nq} seeded random queries of {n} points; reorder "
                           f"{r['reorder_s']:.2f} s + EGO sort {r['sort_s']:.2f} s not in the rate; "
                           f"join {r['join_s']:.2f} s, {r['pairs']} pairs",
                 "join_s": r["join_s"], "queries": nq}
-    threads = os.cpu_count() or 1
     qids = order[:2000]
     tot, t = O.superego_sample(pts, eps, qids, threads)
     return {"value": tot / max(t[2], 1e-9), "unit": "result pairs/s", "cores": threads,
-            "kind": "port", "sample": f"Super-EGO restatement, {len(qids)} queries"}
+            "kind": "port", "cpu_model": model, "host_cpus_visible": visible,
+            "sample": f"Super-EGO restatement, {len(qids)} queries"}
 
 
 def rooflines(n, dim, cells, pairs, evals, kernel_ms, refine):
@@ -119,16 +144,20 @@ def rooflines(n, dim, cells, pairs, evals, kernel_ms, refine):
         peak = PEAK_BF16_TFLOPS / 3.0
         note = ("2 D flops per candidate pair; peak = bf16 dense MFMA / 3 (split-BF16 "
                 "hi.hi + hi.lo + lo.hi); kernel k_gb_join")
+        bound = "mfma"
     elif refine == "f32":
         flops = 3.0 * dim * evals
-        peak = PEAK_FP32_TFLOPS
-        note = ("3 D FP32 flops (sub + fma per dim) per distance evaluation, FP32 vector peak; "
-                "kernel k_join with the FP32 filter")
+        peak = PEAK_FP32_UNPACKED_TFLOPS
+        bound = "valu-fp32"
+        note = ("3 D FP32 flops (sub + fma per dim) per distance evaluation against the FP32 "
+                "vector peak without packed FP32 (157.3 / 2 TFLOP/s; the build does not emit "
+                "packed FP32); kernel k_join with the FP32 filter")
     else:
         flops = 3.0 * dim * evals
         peak = PEAK_FP64_TFLOPS
+        bound = "valu-fp64"
         note = "3 D FP64 flops (sub + fma per dim) per distance evaluation; kernel k_join"
-    comp = {"bound": "mfma", "achieved": flops / t / 1e12, "peak": peak, "unit": "TFLOP/s",
+    comp = {"bound": bound, "achieved": flops / t / 1e12, "peak": peak, "unit": "TFLOP/s",
             "frac": flops / t / 1e12 / peak, "traffic": None, "work": flops,
             "roof_ms": flops / (peak * 1e12) * 1e3, "note": note}
     return (comp, hbm) if comp["roof_ms"] >= hbm["roof_ms"] else (hbm, comp)

@@ -308,6 +308,7 @@ struct Counters {
     unsigned long long evals;
     unsigned long long pairs;
     unsigned long long band;  // FP32 filter: pairs rechecked in FP64
+    unsigned long long mismatch;  // units whose write pass disagreed with their count
 };
 
 hegj_status ensure_morton(hegj_index *h) {
@@ -426,6 +427,7 @@ JoinArgs base_args(const hegj_index *h, const hegj_join_opts &opts, const JoinPl
     a.part_by_xcc = opts.part_by_block ? 0u : 1u;
     a.prune = opts.no_prune ? 0u : 1u;
     a.band = &P.d_ctr->band;
+    a.mismatch = &P.d_ctr->mismatch;
     if (P.f32) {
         a.f32 = 1;
         a.soa32 = h->soa32;
@@ -551,6 +553,9 @@ hegj_status run_batches(hegj_index *h, const hegj_join_opts &opts, uint32_t qb, 
         HEGJ_TRY(launch_pair_hash(a, capacity, s));
         HIP_TRY(hipMemcpyAsync(&c, P.d_ctr, sizeof(c), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
+        if (c.mismatch)
+            return fail(HEGJ_ERR_HIP, "join: the write pass of " + std::to_string(c.mismatch) +
+                                          " work units disagreed with their counts");
         if (r->batches.size() >= (size_t)kMaxBatches && c.n_deferred)
             return fail(HEGJ_ERR_CAPACITY, "result needs more than 64 batches");
         base += capacity;
@@ -713,6 +718,9 @@ hegj_status selfjoin_graph(hegj_index *h, const hegj_join_opts &opts, uint32_t q
         r->info.join_ms = t.stop();  // synchronises the stream
     }
     const Counters c = *G.h_ctr;
+    if (c.mismatch)
+        return fail(HEGJ_ERR_HIP, "join: the write pass of " + std::to_string(c.mismatch) +
+                                      " work units disagreed with their counts");
     if (c.n_deferred) {
         // grow the cached capacity for later replays; this call runs eagerly
         G.P.capacity = std::max<uint64_t>(G.P.capacity * 2, c.cursor + (1u << 20));

@@ -159,6 +159,7 @@ struct JoinArgs {
     const float *aos32;           // [n][dim], cell order (dim <= kAosMaxDim)
     float tacc32, trej32;
     unsigned long long *band;     // pairs rechecked in FP64
+    unsigned long long *mismatch; // units whose write pass disagreed with their count
 };
 
 // ---- index (hegj_index.hip) ----------------------------------------------

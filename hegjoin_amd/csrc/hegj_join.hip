@@ -823,7 +823,19 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())
                 }
             } else {
                 // masks did not fit anywhere (very large units): evaluate
-                // each chunk again, one query at a time
+                // each chunk again, one query at a time.  The writes stay
+                // inside each query's reserved row [oq, oend); a count that
+                // differs from the first pass is reported (a.mismatch), never
+                // written past the row.
+                uint64_t oend[QB];
+                {
+                    const uint64_t e_l = base + excl_l + cnt_l;
+#pragma unroll
+                    for (int qi = 0; qi < QB; qi++)
+                        oend[qi] = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(e_l >> 32), qi) << 32) |
+                                   (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)e_l, qi);
+                }
+                bool bad = false;
                 for (uint32_t ch = 0; ch < nchunks; ch++) {
                     const uint32_t t = ch * 64 + lane;
                     const bool valid = t < M;
@@ -850,10 +862,17 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())
                         }
                         const uint64_t m = __ballot(valid && acc1 <= T);
                         if (m == 0) continue;
-                        if (__builtin_amdgcn_inverse_ballot_w64(m)) gst(a.out + oq[qi] + lanes_below(m), id);
-                        oq[qi] += (uint64_t)__popcll(m);
+                        const uint64_t room = oend[qi] - oq[qi];
+                        const uint64_t pc = (uint64_t)__popcll(m);
+                        bad |= pc > room;
+                        if (__builtin_amdgcn_inverse_ballot_w64(m) && lanes_below(m) < room)
+                            gst(a.out + oq[qi] + lanes_below(m), id);
+                        oq[qi] += pc < room ? pc : room;
                     }
                 }
+#pragma unroll
+                for (int qi = 0; qi < QB; qi++) bad |= (uint32_t)qi < nq && oq[qi] != oend[qi];
+                if (bad && lane == 0) atomicAdd(a.mismatch, 1ull);
             }
         }
     }

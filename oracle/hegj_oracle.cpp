@@ -220,11 +220,92 @@ void join_fill_t(const double *p, uint32_t n, uint32_t dim, uint32_t k, double e
     }
 }
 
+template <typename T>
+uint64_t join_rows_t(const double *p, uint32_t n, uint32_t dim, uint32_t k, double eps,
+                     int emulate_cell0, const uint32_t *qids, uint32_t nq, uint32_t *counts,
+                     const uint64_t *offsets, uint32_t *nbrs) {
+    Grid<T> g(p, n, dim, k, eps);
+    uint64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : total)
+    for (int64_t i = 0; i < (int64_t)nq; i++) {
+        std::vector<uint32_t> row;
+        g.join_query(qids[i], emulate_cell0, [&](uint32_t c) { row.push_back(c); });
+        std::sort(row.begin(), row.end());
+        if (counts) counts[i] = (uint32_t)row.size();
+        if (nbrs) std::copy(row.begin(), row.end(), nbrs + offsets[i]);
+        total += row.size();
+    }
+    return total;
+}
+
 uint64_t mix64(uint64_t x) {
     uint64_t z = x + 0x9E3779B97F4A7C15ull;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return z ^ (z >> 31);
+}
+
+float threshold_of(float eps) { return og_threshold_f(eps); }
+double threshold_of(double eps) { return og_threshold(eps); }
+
+// Count and multiset hash of the algorithm-0 pair set without holding it
+// (the pair sets of the full-size workloads have up to 1.2e9 pairs).  Same
+// semantics as join_query, organised for speed:
+//  - the stencil depends only on the query's cell (kernel.cu:763-782 derive it
+//    from the cell coordinates), so it is enumerated once per non-empty cell;
+//  - candidates are read from a cell-ordered copy of the points;
+//  - sqrt(acc) <= eps is evaluated as acc <= T, T = the largest value whose
+//    sqrt is <= eps (og_threshold; sqrt is monotone);
+//  - the fma chain stops as soon as acc > T: fma(d, d, acc) >= acc for every d
+//    (the exact d*d + acc is >= acc and rounding is monotone), so once above T
+//    the final acc stays above T and the decision is the same.
+template <typename T>
+uint64_t join_stats_t(const double *p, uint32_t n, uint32_t dim, uint32_t k, double eps,
+                      int emulate_cell0, uint64_t *hash, uint32_t *row_cnt, uint64_t *row_hash) {
+    Grid<T> g(p, n, dim, k, eps);
+    const T thr = threshold_of(g.eps);
+    std::vector<T> cp((size_t)n * dim);  // points in cell order
+    for (uint32_t i = 0; i < n; i++)
+        std::memcpy(&cp[(size_t)i * dim], &g.pts[(size_t)g.indexLookupArr[i] * dim], sizeof(T) * dim);
+    const uint32_t C = (uint32_t)g.cellKeys.size();
+    uint64_t total = 0, h_all = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : total, h_all)
+    for (int64_t ci = 0; ci < (int64_t)C; ci++) {
+        std::vector<uint32_t> c(k), st;
+        const uint32_t first = (uint32_t)g.grid[ci].indexmin, last = (uint32_t)g.grid[ci].indexmax;
+        g.coords(&cp[(size_t)first * dim], c.data());
+        g.for_stencil(c.data(), emulate_cell0, [&](uint32_t s) { st.push_back(s); });
+        for (uint32_t qp = first; qp <= last; qp++) {
+            const T *xq = &cp[(size_t)qp * dim];
+            const uint32_t qid = g.indexLookupArr[qp];
+            uint32_t cnt = 0;
+            uint64_t h = 0;
+            for (uint32_t s : st) {
+                for (int32_t m = g.grid[s].indexmin; m <= g.grid[s].indexmax; m++) {
+                    const T *xc = &cp[(size_t)m * dim];
+                    T acc = 0;
+                    uint32_t l = 0;
+                    for (; l < dim; l++) {
+                        const T d = xc[l] - xq[l];
+                        acc = std::fma(d, d, acc);
+                        if (acc > thr) break;
+                    }
+                    if (l == dim) {
+                        cnt++;
+                        h += mix64(((uint64_t)qid << 32) | g.indexLookupArr[m]);
+                    }
+                }
+            }
+            if (row_cnt) row_cnt[qid] = cnt;
+            if (row_hash) row_hash[qid] = h;
+            total += cnt;
+            h_all += h;
+        }
+    }
+    // queries silenced by the cell-0 wrap are in cells whose stencil is empty
+    // and got a zero row above
+    *hash = h_all;
+    return total;
 }
 
 // ---------------------------------------------------------------------------
@@ -585,6 +666,21 @@ uint64_t og_workload(const double *pts, uint32_t n, uint32_t dim, uint32_t k, do
         prec += nb;
     }
     return acc;
+}
+
+uint64_t og_join_stats(const double *pts, uint32_t n, uint32_t dim, uint32_t k, double eps,
+                       int emulate_cell0, int dtype_f32, uint64_t *hash, uint32_t *row_cnt,
+                       uint64_t *row_hash) {
+    if (dtype_f32) return join_stats_t<float>(pts, n, dim, k, eps, emulate_cell0, hash, row_cnt, row_hash);
+    return join_stats_t<double>(pts, n, dim, k, eps, emulate_cell0, hash, row_cnt, row_hash);
+}
+
+uint64_t og_join_rows(const double *pts, uint32_t n, uint32_t dim, uint32_t k, double eps,
+                      int emulate_cell0, int dtype_f32, const uint32_t *qids, uint32_t nq,
+                      uint32_t *counts, const uint64_t *offsets, uint32_t *nbrs) {
+    if (dtype_f32)
+        return join_rows_t<float>(pts, n, dim, k, eps, emulate_cell0, qids, nq, counts, offsets, nbrs);
+    return join_rows_t<double>(pts, n, dim, k, eps, emulate_cell0, qids, nq, counts, offsets, nbrs);
 }
 
 uint64_t og_join_counts(const double *pts, uint32_t n, uint32_t dim, uint32_t k, double eps,

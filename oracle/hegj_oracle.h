@@ -107,6 +107,22 @@ uint64_t og_join_counts(const double *pts, uint32_t n, uint32_t dim, uint32_t k,
 void og_join_fill(const double *pts, uint32_t n, uint32_t dim, uint32_t k, double eps,
                   int emulate_cell0, int dtype_f32, const uint64_t *offsets, uint32_t *nbrs);
 
+/* Count and multiset hash (sum of og_pair_mix(query, neighbour)) of the same
+ * pair set as og_join_counts, without holding the pairs: the stencil is
+ * enumerated once per cell and each fma chain stops once it exceeds the
+ * threshold (the chain never decreases).  row_cnt[n] / row_hash[n] (may be
+ * NULL) receive per-query counts and hashes.  Pinned by equality with
+ * og_join_counts + og_csr_hash (tests/test_oracle.py). */
+uint64_t og_join_stats(const double *pts, uint32_t n, uint32_t dim, uint32_t k, double eps,
+                       int emulate_cell0, int dtype_f32, uint64_t *hash, uint32_t *row_cnt,
+                       uint64_t *row_hash);
+
+/* og_join_counts / og_join_fill for the queries qids[nq] only: counts[nq];
+ * with nbrs, offsets[nq+1] from the counts and rows ascending. */
+uint64_t og_join_rows(const double *pts, uint32_t n, uint32_t dim, uint32_t k, double eps,
+                      int emulate_cell0, int dtype_f32, const uint32_t *qids, uint32_t nq,
+                      uint32_t *counts, const uint64_t *offsets, uint32_t *nbrs);
+
 /* Brute force O(n^2), no grid: pair (q, c) iff sqrt(fma-chain(q,c)) <= eps,
  * computed literally with sqrt; a query whose cell has a zero coordinate in
  * any of the first k dims emits nothing when emulate_cell0.  Same outputs as

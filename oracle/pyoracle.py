@@ -60,6 +60,12 @@ def lib():
         L.og_join_counts.argtypes = [_f64p, u32, u32, u32, f64, i32, i32, _u32p]
         L.og_join_counts.restype = u64
         L.og_join_fill.argtypes = [_f64p, u32, u32, u32, f64, i32, i32, _u64p, _u32p]
+        L.og_join_stats.argtypes = [_f64p, u32, u32, u32, f64, i32, i32, C.POINTER(u64),
+                                    C.c_void_p, C.c_void_p]
+        L.og_join_stats.restype = u64
+        L.og_join_rows.argtypes = [_f64p, u32, u32, u32, f64, i32, i32, _u32p, u32, C.c_void_p,
+                                   C.c_void_p, C.c_void_p]
+        L.og_join_rows.restype = u64
         L.og_brute_counts.argtypes = [_f64p, u32, u32, u32, f64, i32, _u32p]
         L.og_brute_counts.restype = u64
         L.og_brute_fill.argtypes = [_f64p, u32, u32, u32, f64, i32, _u64p, _u32p]
@@ -180,6 +186,69 @@ def join(pts, k, eps, emulate_cell0=True, f32=False):
                 n)
 
 
+def join_stats(pts, k, eps, emulate_cell0=True, f32=False, rows=False):
+    """Algorithm-0 (count, hash) without the pairs; with rows=True also the
+    per-query counts and hashes."""
+    pts, n, d = _pts(pts)
+    h = C.c_uint64()
+    rc = np.zeros(n, np.uint32) if rows else None
+    rh = np.zeros(n, np.uint64) if rows else None
+    total = lib().og_join_stats(pts.reshape(-1), n, d, k, eps, int(emulate_cell0), int(f32),
+                                C.byref(h), rc.ctypes.data if rows else None,
+                                rh.ctypes.data if rows else None)
+    if rows:
+        return int(total), h.value, rc, rh
+    return int(total), h.value
+
+
+def join_rows(pts, k, eps, qids, emulate_cell0=True, f32=False):
+    """Algorithm-0 rows of the queries qids only: CSR over qids, rows ascending."""
+    pts, n, d = _pts(pts)
+    qids = np.ascontiguousarray(qids, np.uint32)
+    flat = pts.reshape(-1)
+    L = lib()
+    counts = np.zeros(max(len(qids), 1), np.uint32)
+    total = L.og_join_rows(flat, n, d, k, eps, int(emulate_cell0), int(f32), qids, len(qids),
+                           counts.ctypes.data, None, None)
+    offsets = np.zeros(len(qids) + 1, np.uint64)
+    np.cumsum(counts[:len(qids)], out=offsets[1:])
+    nbrs = np.zeros(max(int(total), 1), np.uint32)
+    L.og_join_rows(flat, n, d, k, eps, int(emulate_cell0), int(f32), qids, len(qids), None,
+                   offsets.ctypes.data, nbrs.ctypes.data)
+    return offsets, nbrs[:total]
+
+
+def ref_superego_rowstats(pts, eps):
+    """Reference Util::egoJoinV2 per query: (counts[n], per-row hashes[n])."""
+    pts, n, d = _pts(pts)
+    with tempfile.TemporaryDirectory() as td:
+        pin = os.path.join(td, "p.bin")
+        out = os.path.join(td, "o.bin")
+        pts.tofile(pin)
+        subprocess.run([os.path.join(REF_DIR, f"ref_harness_d{d}"), "rowstats", pin, repr(eps),
+                        out], check=True, capture_output=True)
+        raw = np.fromfile(out, np.uint8)
+    return raw[:4 * n].view(np.uint32).copy(), raw[4 * n:].view(np.uint64).copy()
+
+
+def ref_superego_rows(pts, eps, qids):
+    """Reference Util::egoJoinV2 rows of the queries qids: CSR over qids."""
+    pts, n, d = _pts(pts)
+    with tempfile.TemporaryDirectory() as td:
+        pin = os.path.join(td, "p.bin")
+        qin = os.path.join(td, "q.bin")
+        out = os.path.join(td, "o.bin")
+        pts.tofile(pin)
+        np.ascontiguousarray(qids, np.uint32).tofile(qin)
+        subprocess.run([os.path.join(REF_DIR, f"ref_harness_d{d}"), "rows", pin, repr(eps), qin,
+                        out], check=True, capture_output=True)
+        raw = np.fromfile(out, np.uint32)
+    nq = len(qids)
+    offsets = np.zeros(nq + 1, np.uint64)
+    np.cumsum(raw[:nq], out=offsets[1:])
+    return offsets, raw[nq:].copy()
+
+
 def brute(pts, k, eps, emulate_cell0=True):
     pts, n, d = _pts(pts)
     flat = pts.reshape(-1)
@@ -282,18 +351,22 @@ def ref_superego_pairs(pts, eps):
     return offsets, raw[n:].copy()
 
 
-def ref_superego_time(pts, eps, order, nq, workdir=None):
-    """Times the reference's multiThreadJoinWorkQueue over order[:nq].  Returns
-    the parsed REF_TIME dict."""
+def ref_superego_time(pts, eps, order, nq, workdir=None, threads=None):
+    """Times the reference's multiThreadJoinWorkQueue over order[:nq] on
+    `threads` threads (its CPU_THREADS; default: the harness's own choice,
+    OMP_NUM_THREADS or the process's CPUs).  Returns the parsed REF_TIME dict."""
     import json
     pts, n, d = _pts(pts)
+    env = dict(os.environ)
+    if threads:
+        env["HEGJ_REF_THREADS"] = str(int(threads))
     with tempfile.TemporaryDirectory(dir=workdir) as td:
         pin = os.path.join(td, "p.bin")
         oin = os.path.join(td, "order.bin")
         pts.tofile(pin)
         np.ascontiguousarray(order, np.uint32).tofile(oin)
         r = subprocess.run([os.path.join(REF_DIR, f"ref_harness_d{d}"), "time", pin, repr(eps),
-                            oin, str(nq)], check=True, capture_output=True, text=True)
+                            oin, str(nq)], check=True, capture_output=True, text=True, env=env)
     for line in r.stdout.splitlines():
         if line.startswith("REF_TIME "):
             return json.loads(line[len("REF_TIME "):])

@@ -13,13 +13,20 @@
 //   time   -- Util::multiThreadJoinWorkQueue (MultiThreadJoin.cpp:22) in
 //             mode SM_CPU over the first nq entries of a query order given
 //             in a file (a seeded sample for a bounded baseline run);
-//   load   -- importNDDataset (import_dataset.cpp:10) and dump the doubles.
+//   load   -- importNDDataset (import_dataset.cpp:10) and dump the doubles;
+//   rowstats -- the same per-query egoJoinV2 as `pairs`, keeping per query
+//             only its neighbour count and the multiset hash
+//             sum mix64(query << 32 | neighbour) (the full-size workloads have
+//             up to 1.2e9 pairs);
+//   rows   -- `pairs` for a list of query ids only.
 //
 // Usage:
 //   ref_harness_dD load  <dataset.txt> <out.bin>
 //   ref_harness_dD pairs <points.bin> <eps> <out.bin>
 //   ref_harness_dD time  <points.bin> <eps> <order.bin> <nq>
-// points.bin = raw row-major doubles (n*D); order.bin = uint32 query ids.
+//   ref_harness_dD rowstats <points.bin> <eps> <out.bin>   (n x u32 counts, n x u64 hashes)
+//   ref_harness_dD rows <points.bin> <eps> <qids.bin> <out.bin>
+// points.bin = raw row-major doubles (n*D); order.bin / qids.bin = uint32 ids.
 #include <algorithm>
 #include <cstdint>
 #include <cstdio>
@@ -37,6 +44,13 @@
 #include "structs.h"
 
 void importNDDataset(std::vector<std::vector<DTYPE> > *dataPoints, char *fname);
+
+static uint64_t mix64(uint64_t x) {
+    uint64_t z = x + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
 
 static std::vector<double> read_bin(const char *path) {
     FILE *f = std::fopen(path, "rb");
@@ -120,6 +134,55 @@ int main(int argc, char **argv) {
             std::fwrite(&c, 4, 1, f);
         }
         for (unsigned q = 0; q < n; q++) std::fwrite(rows[q].data(), 4, rows[q].size(), f);
+        std::fclose(f);
+        return 0;
+    }
+    if (mode == "rowstats") {
+        std::vector<uint32_t> cnt(n);
+        std::vector<uint64_t> hsh(n);
+#pragma omp parallel
+        {
+            std::vector<int> row;
+#pragma omp for schedule(dynamic, 64)
+            for (long q = 0; q < (long)n; q++) {
+                uint64_t cand = 0;
+                int index = (int)mapping[q];
+                row.clear();
+                Util::egoJoinV2(A, 0, (int)n - 1, A, index, index, 0, &cand, &row, nullptr);
+                uint64_t h = 0;
+                for (int c : row) h += mix64(((uint64_t)q << 32) | (uint32_t)c);
+                cnt[q] = (uint32_t)row.size();
+                hsh[q] = h;
+            }
+        }
+        FILE *f = std::fopen(argv[4], "wb");
+        std::fwrite(cnt.data(), 4, n, f);
+        std::fwrite(hsh.data(), 8, n, f);
+        std::fclose(f);
+        return 0;
+    }
+    if (mode == "rows" && argc >= 6) {
+        FILE *fq = std::fopen(argv[4], "rb");
+        std::fseek(fq, 0, SEEK_END);
+        long nq = std::ftell(fq) / 4;
+        std::fseek(fq, 0, SEEK_SET);
+        std::vector<uint32_t> qids(nq);
+        if (std::fread(qids.data(), 4, nq, fq) != (size_t)nq) return 2;
+        std::fclose(fq);
+        std::vector<std::vector<int> > rows(nq);
+#pragma omp parallel for schedule(dynamic, 1)
+        for (long i = 0; i < nq; i++) {
+            uint64_t cand = 0;
+            int index = (int)mapping[qids[i]];
+            Util::egoJoinV2(A, 0, (int)n - 1, A, index, index, 0, &cand, &rows[i], nullptr);
+            std::sort(rows[i].begin(), rows[i].end());
+        }
+        FILE *f = std::fopen(argv[5], "wb");
+        for (long i = 0; i < nq; i++) {
+            uint32_t c = (uint32_t)rows[i].size();
+            std::fwrite(&c, 4, 1, f);
+        }
+        for (long i = 0; i < nq; i++) std::fwrite(rows[i].data(), 4, rows[i].size(), f);
         std::fclose(f);
         return 0;
     }

@@ -233,10 +233,13 @@ def test_bench_rooflines_pick_the_binding_roof():
     B = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(B)
     roof, other = B.rooflines(1_000_000, 2, 20_000, 1.06e9, 3.0e9, 10.0, "fp64")
-    assert roof["bound"] == "hbm" and other["bound"] == "mfma"
+    assert roof["bound"] == "hbm" and other["bound"] == "valu-fp64"
     assert abs(roof["work"] - (8 * 2e6 + 4e6 + 24 * 2e4 + 8 * 1.06e9)) < 1
     roof, other = B.rooflines(5_000_000, 6, 1_860_000, 3.0e8, 2.4e10, 100.0, "fp64")
-    assert roof["bound"] == "mfma" and abs(roof["peak"] - 78.6) < 1e-9
+    assert roof["bound"] == "valu-fp64" and abs(roof["peak"] - 78.6) < 1e-9
+    # the FP32 filter: unpacked FP32 VALU (the build emits no packed FP32)
+    roof, other = B.rooflines(5_000_000, 6, 1_860_000, 3.0e8, 2.4e10, 100.0, "f32")
+    assert roof["bound"] == "valu-fp32" and abs(roof["peak"] - 157.3 / 2) < 1e-9
     assert abs(roof["achieved"] - 3 * 6 * 2.4e10 / 0.1 / 1e12) < 1e-9
     roof, other = B.rooflines(2_000_000, 18, 2900, 7.4e7, 2.8e12, 500.0, "mfma")
     assert roof["bound"] == "mfma" and abs(roof["peak"] - 2500 / 3) < 1e-9

@@ -325,6 +325,58 @@ def test_large_units_use_global_masks(H, oracle, n, eps):
     assert np.array_equal(off, eoff) and np.array_equal(nb, enb)
 
 
+def dense_slab(dim, n, eps, width, seed):
+    """A slab three cells wide along dim 0 (k = 1: every point of the slab is
+    a candidate of every query in it), the other dims uniform over `width`,
+    plus partners of 300 points at distance eps (1 + r), r in {+-1e-9,
+    +-1e-15, 0}, which the FP32 filter can only settle by the FP64 recheck."""
+    rng = np.random.default_rng(seed)
+    pts = np.empty((n, dim))
+    pts[:, 0] = 0.5 + 2.5 * eps * rng.random(n)
+    pts[:, 1:] = 0.3 + width * rng.random((n, dim - 1))
+    d = rng.standard_normal((300, dim))
+    d[:, 0] = 0.0  # partners stay in the slab's cells
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rel = rng.choice([-1e-9, -1e-15, 0.0, 1e-15, 1e-9], 300)
+    part = pts[:300] + d * (eps * (1 + rel))[:, None]
+    return np.ascontiguousarray(np.concatenate([pts, part]))
+
+
+# (dim, n, eps, width, queries per unit, expected mask store): with M slab
+# candidates a unit keeps qb * ceil(M / 64) hit masks; above 512 they go to
+# the per-wave global scratch, above 8192 the write pass re-evaluates the
+# distances instead
+_DENSE_CACHE = {}
+DENSE = [(2, 40000, 0.004, 1.0, 8, "global"), (2, 40000, 0.004, 1.0, 16, "reevaluate"),
+         (6, 70000, 0.01, 0.05, 4, "global"), (6, 70000, 0.01, 0.05, 8, "reevaluate")]
+
+
+@pytest.mark.parametrize("refine", ["f32", "fp64"])
+@pytest.mark.parametrize("dim,n,eps,width,qb,store", DENSE,
+                         ids=[f"d{c[0]}-qb{c[4]}-{c[5]}" for c in DENSE])
+def test_dense_units_mask_stores(H, oracle, dim, n, eps, width, qb, store, refine):
+    """The register kernels (D = 2, 6) with the FP32 filter and the FP64
+    chain on units whose masks exceed LDS: the global-scratch path (with FP32
+    band rechecks OR-ed into global masks) and the re-evaluating write pass
+    (its rows must hold exactly the counted pairs), both against the oracle."""
+    print("case", dim, n, qb, store, refine, flush=True)
+    if dim not in _DENSE_CACHE:
+        pts = dense_slab(dim, n, eps, width, seed=dim * 100)
+        # count, pair hash and per-query counts (the pair set has millions of
+        # pairs; the hash covers every one of them)
+        _DENSE_CACHE[dim] = (pts, oracle.join_stats(pts, 1, eps, rows=True))
+    pts, (total, phash, rcnt, _) = _DENSE_CACHE[dim]
+    with H.GridIndex(pts, eps, indexed_dims=1) as ix:
+        t = ix.selfjoin(refine=refine, queries_per_unit=qb, prune=False)
+        info, counts = t.info, t.counts()
+        t.close()
+    assert info["refine_used"] == refine
+    if refine == "f32":
+        assert info["band_rechecks"] > 0
+    assert info["total_neighbors"] == total and info["pair_hash"] == phash
+    assert np.array_equal(counts, rcnt)
+
+
 def test_duplicate_points_and_tiny_inputs(H, oracle):
     """Duplicates (distance 0), a single point, and all points in one cell."""
     for pts, eps in [(np.full((50, 3), 0.5), 0.1), (np.array([[0.3, 0.7]]), 0.2),

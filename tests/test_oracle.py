@@ -253,6 +253,58 @@ def test_float_dtype_join_equals_f32_brute(oracle):
     assert mism == 0
 
 
+@pytest.mark.parametrize("dim,k,n,eps,f32,emulate", [
+    (2, 2, 6000, 0.004, False, True), (6, 6, 8000, 0.03, False, True), (6, 3, 5000, 0.05, True, True),
+    (18, 6, 1500, 0.2, False, True), (3, 1, 4000, 0.01, False, False)])
+def test_join_stats_equals_join(oracle, dim, k, n, eps, f32, emulate):
+    """The streaming count + hash (stencil once per cell, fma chain stopped
+    once above the threshold) equals the per-query join's CSR and its hash,
+    per query and in total; join_rows gives the same rows for a subset."""
+    pts = oracle.generate_expo(n, dim, seed=n + dim)
+    off, nb = oracle.join(pts, k, eps, emulate_cell0=emulate, f32=f32)
+    tot, h, rc, rh = oracle.join_stats(pts, k, eps, emulate_cell0=emulate, f32=f32, rows=True)
+    assert tot == len(nb) and h == oracle.csr_hash(off, nb)
+    assert np.array_equal(rc, np.diff(off))
+    for q in (0, n // 3, n - 1):
+        row = nb[off[q]:off[q + 1]]
+        assert rh[q] == sum(int(oracle.pair_mix(q, int(c))) for c in row) % 2 ** 64
+    qids = np.array([n - 1, 7, 0, n // 2], np.uint32)
+    o2, n2 = oracle.join_rows(pts, k, eps, qids, emulate_cell0=emulate, f32=f32)
+    for i, q in enumerate(qids):
+        assert np.array_equal(n2[o2[i]:o2[i + 1]], nb[off[q]:off[q + 1]])
+
+
+def test_join_stats_planted_mistake_is_caught(oracle):
+    """A strict < in place of <= changes the stats on a lattice with exact
+    distances eps (the early exit must not hide the boundary)."""
+    g = np.arange(10) / 8.0
+    pts = np.array(np.meshgrid(g, g, indexing="ij")).reshape(2, -1).T.copy()
+    tot, _ = oracle.join_stats(pts, 2, 0.125, emulate_cell0=False)
+    # every lattice point: itself + its 2..4 axis neighbours at exactly eps
+    axis = sum(min(i, 1) + min(9 - i, 1) for i in range(10)) * 10 * 2
+    assert tot == 100 + axis
+    assert oracle.join_stats(pts, 2, np.nextafter(0.125, 0), emulate_cell0=False)[0] == 100
+
+
+def test_fullsize_fixture_is_consistent():
+    """tests/golden/fullsize.json: one entry per self-join workload, and at
+    2-D the reference's algorithm 3 differs from algorithm 0 only by pairs
+    the cell-0 wrap or the eps boundary explains."""
+    import json
+    with open(os.path.join(GOLDEN, "fullsize.json")) as f:
+        full = json.load(f)
+    assert {"2d", "6d"} <= set(full)
+    for key, e in full.items():
+        assert e["pairs"] >= e["n"] - e["cell0_silenced"]  # self pairs at least
+        assert len(e["pair_hash"]) == 16
+    a3 = full["2d"]["alg3"]
+    assert a3["attribution"]["unexplained_pairs"] == 0
+    diff = a3["attribution"]["only_alg0"] + a3["attribution"]["only_alg3"]
+    assert diff == a3["attribution"]["cell0_pairs"] + a3["attribution"]["boundary_pairs"]
+    assert (a3["pairs"] - full["2d"]["pairs"]
+            == a3["attribution"]["only_alg3"] - a3["attribution"]["only_alg0"])
+
+
 # --------------------------------------------------------------------------
 # workload (sortByWorkLoad)
 # --------------------------------------------------------------------------
