@@ -75,7 +75,24 @@ class _ResultInfo(C.Structure):
                 ("estimate_ms", C.c_double), ("join_ms", C.c_double), ("workload_ms", C.c_double),
                 ("total_ms", C.c_double), ("join_kernel_avg_ms", C.c_double),
                 ("join_launches", C.c_uint32), ("refine_used", C.c_uint32),
-                ("graph_replay", C.c_uint32), ("reserved", C.c_uint32)]
+                ("graph_replay", C.c_uint32), ("reserved", C.c_uint32),
+                ("estimated_pairs", C.c_uint64)]
+
+
+class _HybridOpts(C.Structure):
+    _fields_ = [("mode", C.c_uint32), ("cpu_threads", C.c_uint32), ("cpu_batch", C.c_uint32),
+                ("gpu_chunk", C.c_uint32), ("gpu_fraction", C.c_double)]
+
+
+class _HybridInfo(C.Structure):
+    _fields_ = [("mode", C.c_uint32), ("cpu_threads", C.c_uint32), ("n_queries_gpu", C.c_uint32),
+                ("n_queries_cpu", C.c_uint32), ("pairs_gpu", C.c_uint64), ("pairs_cpu", C.c_uint64),
+                ("candidates_cpu", C.c_uint64), ("gpu_fraction", C.c_double), ("gpu_s", C.c_double),
+                ("cpu_s", C.c_double), ("reorder_s", C.c_double), ("sort_s", C.c_double),
+                ("gpu_end_s", C.c_double), ("cpu_end_s", C.c_double)]
+
+
+MODES = {"gpu": 0, "hybrid": 1, "static": 2, "cpu": 3}
 
 
 class _ServeOpts(C.Structure):
@@ -121,6 +138,8 @@ _lib.hegj_version.restype = C.c_char_p
 _lib.hegj_dataset_generate_expo.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, _vp]
 _lib.hegj_dataset_load.argtypes = [C.c_char_p, C.c_uint32, C.POINTER(C.POINTER(C.c_double)),
                                    C.POINTER(C.c_uint32)]
+_lib.hegj_dataset_load_ex.argtypes = [C.c_char_p, C.c_uint32, C.c_uint32,
+                                      C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.c_uint32)]
 _lib.hegj_dataset_free.argtypes = [_vp]
 _lib.hegj_stream_copy_bandwidth.argtypes = [C.c_uint64, C.c_int, C.POINTER(C.c_double)]
 _lib.hegj_release_cached_memory.argtypes = [C.c_int]
@@ -144,6 +163,20 @@ _lib.hegj_f32_thresholds.argtypes = [C.c_double, C.c_uint32, C.c_double, C.POINT
 _lib.hegj_join_opts_tuned.argtypes = [C.POINTER(_JoinOpts), C.c_uint32]
 _lib.hegj_tuning_set.argtypes = [C.c_uint32, C.POINTER(_JoinOpts)]
 _lib.hegj_tuning_load.argtypes = [C.c_char_p, C.POINTER(C.c_uint32)]
+_lib.hegj_hybrid_opts_default.argtypes = [C.POINTER(_HybridOpts), C.c_uint32]
+_lib.hegj_hybrid_opts_default.restype = None
+_lib.hegj_selfjoin_hybrid.argtypes = [_vp, C.POINTER(_JoinOpts), C.POINTER(_HybridOpts),
+                                      C.POINTER(_vp)]
+_lib.hegj_result_get_hybrid_info.argtypes = [_vp, C.POINTER(_HybridInfo)]
+_lib.hegj_superego_rows.argtypes = [_vp, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32, _vp,
+                                    C.c_uint32, _vp, C.POINTER(C.POINTER(C.c_uint32)),
+                                    C.POINTER(C.c_uint64)]
+_lib.hegj_free.argtypes = [_vp]
+_lib.hegj_free.restype = None
+_lib.hegj_static_partition_model.argtypes = [C.c_uint32, C.c_uint32, C.c_double]
+_lib.hegj_static_partition_model.restype = C.c_double
+_lib.hegj_static_partition_set_model.argtypes = [_vp, _vp]
+_lib.hegj_static_partition_set_model.restype = None
 _lib.hegj_serve_open.argtypes = [_vp, C.c_uint32, C.POINTER(_ServeOpts), C.POINTER(_vp)]
 _lib.hegj_serve_submit.argtypes = [_vp, _vp, C.c_uint32, C.c_double, C.POINTER(C.c_uint64)]
 _lib.hegj_serve_wait.argtypes = [_vp, C.c_uint64, C.POINTER(_RequestInfo)]
@@ -213,16 +246,56 @@ def generate_expo(n, dim, seed=0, lam=40.0):
     return out
 
 
-def load_dataset(path, dim):
-    """importNDDataset-compatible text loader (*.bin: raw row-major doubles)."""
+def load_dataset(path, dim, f32=False, bin_cache=False):
+    """importNDDataset-compatible text loader (*.bin: raw row-major doubles).
+    f32: parse as DTYPE=float; bin_cache: read / write the binary cache
+    <path>.d<dim>[f].hegjbin beside a text file."""
     p = C.POINTER(C.c_double)()
     n = C.c_uint32()
-    _check(_lib.hegj_dataset_load(str(path).encode(), dim, C.byref(p), C.byref(n)))
+    flags = (1 if bin_cache else 0) | (2 if f32 else 0)
+    _check(_lib.hegj_dataset_load_ex(str(path).encode(), dim, flags, C.byref(p), C.byref(n)))
     try:
         return np.ctypeslib.as_array(p, shape=(n.value * dim,)).copy().reshape(n.value, dim) \
             if n.value else np.zeros((0, dim))
     finally:
         _lib.hegj_dataset_free(p)
+
+
+def superego_rows(database, epsilon, qids=None, threads=0):
+    """Super-EGO on the host (Util::egoJoinV2 per query, the CPU share of the
+    hybrid modes): CSR (offsets, neighbours) over qids (default: every point),
+    each row in EGO order."""
+    arr = np.ascontiguousarray(database, dtype=np.float64)
+    n, dim = arr.shape
+    q = None if qids is None else np.ascontiguousarray(qids, np.uint32)
+    nq = n if q is None else len(q)
+    counts = np.zeros(nq, np.uint32)
+    p = C.POINTER(C.c_uint32)()
+    tot = C.c_uint64()
+    _check(_lib.hegj_superego_rows(arr.ctypes.data, n, dim, float(epsilon), threads,
+                                   None if q is None else q.ctypes.data, nq, counts.ctypes.data,
+                                   C.byref(p), C.byref(tot)))
+    try:
+        nb = np.ctypeslib.as_array(p, shape=(tot.value,)).copy() if tot.value else \
+            np.zeros(0, np.uint32)
+    finally:
+        _lib.hegj_free(p)
+    offsets = np.zeros(nq + 1, np.uint64)
+    np.cumsum(counts, out=offsets[1:])
+    return offsets, nb
+
+
+def static_partition_model(n, dim, epsilon):
+    """The modelled GPU share of the queries (StaticPartition.cpp)."""
+    return _lib.hegj_static_partition_model(n, dim, float(epsilon))
+
+
+def set_static_partition_model(gpu, cpu):
+    """Replace the (c0, c1, c2, c3) coefficients of the GPU and CPU time models."""
+    g = np.ascontiguousarray(gpu, np.float64)
+    c = np.ascontiguousarray(cpu, np.float64)
+    assert g.shape == (4,) and c.shape == (4,)
+    _lib.hegj_static_partition_set_model(g.ctypes.data, c.ctypes.data)
 
 
 class GridIndex:
@@ -234,7 +307,9 @@ class GridIndex:
     """
 
     def __init__(self, database, epsilon, indexed_dims=None, emulate_cell0=True, device=0,
-                 n=None, dim=None):
+                 n=None, dim=None, dtype="f64"):
+        """dtype: "f64" (DTYPE=double) or "f32" (DTYPE=float: coordinates and
+        epsilon rounded to float, grid and distances in FP32, params.h:24)."""
         self._h = _vp()
         if isinstance(database, int):
             if n is None or dim is None:
@@ -254,9 +329,11 @@ class GridIndex:
             o.indexed_dims = indexed_dims
         o.emulate_cell0 = 1 if emulate_cell0 else 0
         o.device = device
+        o.dtype = {"f64": 0, "f32": 1}[dtype]
         _check(build(ptr, n, float(epsilon), C.byref(o), C.byref(self._h)))
         self._keep = None
         self.n, self.dim = int(n), int(dim)
+        self.dtype = dtype
 
     def close(self):
         if self._h:
@@ -324,6 +401,34 @@ class GridIndex:
         hipGraph of the batch (repeated joins with the same options);
         prune=False evaluates every point of every stencil cell (the pair set
         is the same either way)."""
+        o = self._join_opts(result_capacity, unit_order, queries_per_unit, xcd_parts, refine,
+                            units_per_grab, part_by_block, use_graph, prune)
+        r = _vp()
+        _check(_lib.hegj_selfjoin(self._h, C.byref(o), C.byref(r)))
+        return NeighborTable(r, self)
+
+    def selfjoin_hybrid(self, mode="hybrid", gpu_fraction=0.0, cpu_threads=0, cpu_batch=0,
+                        gpu_chunk=0, **join_kw):
+        """Algorithms 1-3 (hegj_selfjoin_hybrid): mode "hybrid" (the GPU
+        takes units from the head of the workload-ordered queue, Super-EGO CPU
+        threads batches from its tail), "static" (the GPU takes the first
+        gpu_fraction of the queries; <= 0: the model) or "cpu" (Super-EGO
+        alone).  join_kw as for selfjoin.  The result's hybrid_info has the
+        GPU/CPU split."""
+        o = self._join_opts(grid_only=True, **join_kw)
+        h = _HybridOpts()
+        _lib.hegj_hybrid_opts_default(C.byref(h), MODES[mode])
+        h.gpu_fraction = gpu_fraction
+        h.cpu_threads = cpu_threads
+        h.cpu_batch = cpu_batch
+        h.gpu_chunk = gpu_chunk
+        r = _vp()
+        _check(_lib.hegj_selfjoin_hybrid(self._h, C.byref(o), C.byref(h), C.byref(r)))
+        return NeighborTable(r, self)
+
+    def _join_opts(self, result_capacity=0, unit_order=None, queries_per_unit=None,
+                   xcd_parts=None, refine=None, units_per_grab=None, part_by_block=0,
+                   use_graph=False, prune=True, grid_only=False):
         o = _JoinOpts()
         _lib.hegj_join_opts_tuned(C.byref(o), self.dim)
         o.result_capacity = result_capacity
@@ -335,14 +440,14 @@ class GridIndex:
             o.xcd_parts = xcd_parts
         if refine is not None:
             o.refine = {"auto": 0, "fp64": 1, "mfma": 2, "f32": 3}[refine]
+        elif o.refine == 2 and (self.dtype == "f32" or grid_only):
+            o.refine = 0  # the tuned MFMA refine is for DTYPE=double self-joins
         if units_per_grab is not None:
             o.units_per_grab = units_per_grab
         o.part_by_block = part_by_block
         o.use_graph = 1 if use_graph else 0
         o.no_prune = 0 if prune else 1
-        r = _vp()
-        _check(_lib.hegj_selfjoin(self._h, C.byref(o), C.byref(r)))
-        return NeighborTable(r, self)
+        return o
 
 
 class NeighborTable:
@@ -369,6 +474,13 @@ class NeighborTable:
         ri = _ResultInfo()
         _check(_lib.hegj_result_get_info(self._h, C.byref(ri)))
         return _struct_dict(ri)
+
+    @property
+    def hybrid_info(self) -> dict:
+        """GPU / CPU split of a hegj_selfjoin_hybrid result."""
+        hi = _HybridInfo()
+        _check(_lib.hegj_result_get_hybrid_info(self._h, C.byref(hi)))
+        return _struct_dict(hi)
 
     def counts(self):
         c = np.zeros(self.n, np.uint32)

@@ -145,6 +145,15 @@ double sqrt_threshold(double eps) {
     return s;
 }
 
+// DTYPE=float (kernel.cu:363 with DTYPE float: sqrtf, correctly rounded).
+float sqrt_threshold_f(float eps) {
+    float s = eps * eps;
+    const float inf = std::numeric_limits<float>::infinity();
+    while (s > 0 && std::sqrt(s) > eps) s = std::nextafter(s, 0.0f);
+    while (std::sqrt(std::nextafter(s, inf)) <= eps) s = std::nextafter(s, inf);
+    return s;
+}
+
 struct EventTimer {
     hipEvent_t a = nullptr, b = nullptr;
     hipStream_t s;
@@ -211,6 +220,8 @@ struct hegj_result {
     uint64_t *d_row_off = nullptr;
     uint32_t *d_row_cnt = nullptr;
     hegj_result_info info{};
+    bool hybrid = false;        // made by hegj_selfjoin_hybrid
+    hegj_hybrid_info hinfo{};
     hipEvent_t done = nullptr;  // recorded when the join handed the result out
     ~hegj_result() {
         // the buffers go back to the cache: nothing may still be writing them
@@ -244,7 +255,8 @@ hegj_status check_opts(const hegj_index_opts *o, uint32_t n, double eps) {
     if (o->dim == 0) return fail(HEGJ_ERR_INVALID_ARG, "dim must be >= 1");
     if (o->indexed_dims == 0 || o->indexed_dims > HEGJ_MAX_INDEXED_DIMS || o->indexed_dims > o->dim)
         return fail(HEGJ_ERR_INVALID_ARG, "indexed_dims must be in [1, min(dim, 6)]");
-    if (o->dtype != HEGJ_DTYPE_F64) return fail(HEGJ_ERR_UNSUPPORTED, "only DTYPE=double is built");
+    if (o->dtype != HEGJ_DTYPE_F64 && o->dtype != HEGJ_DTYPE_F32)
+        return fail(HEGJ_ERR_INVALID_ARG, "dtype must be HEGJ_DTYPE_F64 or HEGJ_DTYPE_F32");
     if (!(eps > 0.0) || eps > 1.0) return fail(HEGJ_ERR_INVALID_ARG, "epsilon must be in ]0, 1]");
     return HEGJ_OK;
 }
@@ -253,8 +265,15 @@ hegj_status build_common(hegj_index *h, uint32_t n, double eps, const hegj_index
     h->g.n = n;
     h->g.dim = o->dim;
     h->g.k = o->indexed_dims;
-    h->g.eps = eps;
-    h->g.thresh = sqrt_threshold(eps);
+    h->g.f32dt = o->dtype == HEGJ_DTYPE_F32 ? 1u : 0u;
+    if (h->g.f32dt) {  // DTYPE epsilon = atof(...) (main.cu:56) rounds to float
+        const float epsf = (float)eps;
+        h->g.eps = (double)epsf;
+        h->g.thresh = (double)sqrt_threshold_f(epsf);
+    } else {
+        h->g.eps = eps;
+        h->g.thresh = sqrt_threshold(eps);
+    }
     h->g.emulate_cell0 = o->emulate_cell0 ? 1u : 0u;
     EventTimer t(h->stream);
     HEGJ_TRY(build_index(&h->g, h->maxArr, &h->totalCells, &h->ix, &h->n_dead, h->stream));
@@ -365,7 +384,8 @@ hegj_status ensure_soa32(hegj_index *h) {
 // order, their split into per-XCD ranges, the kernel's scratch and the batch
 // capacity (estimated on the device unless the caller fixed it).
 struct JoinPlan {
-    uint2 *units = nullptr, *units_next = nullptr;
+    uint2 *units = nullptr;
+    uint2 *units_next = nullptr, *units_tmp = nullptr;  // deferred units (ping-pong)
     uint32_t U = 0;
     uint32_t *deferred = nullptr;
     Counters *d_ctr = nullptr;
@@ -377,6 +397,7 @@ struct JoinPlan {
     uint32_t part_end[kMaxParts] = {};
     uint64_t capacity = 0;
     double estimate_ms = 0;
+    uint64_t estimated_pairs = 0;  // the estimator's extrapolated pair count
     bool f32 = false;  // FP32 filter with FP64 band recheck
     float tacc32 = 0, trej32 = 0;
     hipStream_t stream = nullptr;  // the stream its kernels run on
@@ -385,8 +406,8 @@ struct JoinPlan {
     JoinPlan &operator=(const JoinPlan &) = delete;
     ~JoinPlan() {
         if (stream) (void)hipStreamSynchronize(stream);
-        for (void *p : {(void *)units, (void *)units_next, (void *)deferred, (void *)d_ctr,
-                        (void *)d_ctr2, (void *)gmask})
+        for (void *p : {(void *)units, (void *)units_next, (void *)units_tmp, (void *)deferred,
+                        (void *)d_ctr, (void *)d_ctr2, (void *)gmask})
             dev_free(p);
     }
 };
@@ -438,14 +459,19 @@ JoinArgs base_args(const hegj_index *h, const hegj_join_opts &opts, const JoinPl
     return a;
 }
 
+// all_cells: units also for the cells the cell-0 wrap silences (the hybrid
+// queue covers every query).
 hegj_status make_plan(hegj_index *h, const hegj_join_opts &opts, uint32_t qb, const uint32_t *order,
-                      JoinPlan &P) {
+                      JoinPlan &P, bool all_cells = false) {
     hipStream_t s = h->stream;
     if (opts.refine == HEGJ_REFINE_F32 || opts.refine == HEGJ_REFINE_AUTO) {
         bool ok = f32_supported(h->g.dim, qb);
         if (ok) {
             HEGJ_TRY(ensure_soa32(h));
-            ok = f32_thresholds(h->g.thresh, h->g.dim, h->f32_radius, &P.tacc32, &P.trej32);
+            if (h->g.f32dt)  // the float chain itself: one threshold, no band
+                P.tacc32 = P.trej32 = (float)h->g.thresh;
+            else
+                ok = f32_thresholds(h->g.thresh, h->g.dim, h->f32_radius, &P.tacc32, &P.trej32);
         }
         if (!ok && opts.refine == HEGJ_REFINE_F32)
             return fail(HEGJ_ERR_UNSUPPORTED,
@@ -454,13 +480,13 @@ hegj_status make_plan(hegj_index *h, const hegj_join_opts &opts, uint32_t qb, co
         P.f32 = ok;
     }
     P.stream = s;
-    HEGJ_TRY(build_units(h->g, h->ix, qb, order, &P.units, &P.U, s));
+    HEGJ_TRY(build_units(h->g, h->ix, qb, order, &P.units, &P.U, s, all_cells));
     const uint32_t U = P.U;
     HIP_TRY(dev_malloc_n(&P.units_next, (size_t)U + 1));
     HIP_TRY(dev_malloc_n(&P.deferred, (size_t)U + 1));
     HIP_TRY(dev_malloc_n(&P.d_ctr, 1));
     HIP_TRY(dev_malloc_n(&P.d_ctr2, 1));
-    HEGJ_TRY(join_occupancy(h->g.dim, qb, h->device, &P.max_blocks));
+    HEGJ_TRY(join_occupancy(h->g.dim, qb, h->device, &P.max_blocks, h->g.f32dt && !P.f32));
     // per-wave hit-mask scratch for units whose masks exceed LDS
     HIP_TRY(dev_malloc_n(&P.gmask, (size_t)P.max_blocks * 4 * P.gmask_slots));
     // unit ranges per XCD, split by estimated work
@@ -492,6 +518,7 @@ hegj_status make_plan(hegj_index *h, const hegj_join_opts &opts, uint32_t qb, co
         HIP_TRY(hipMemcpyAsync(&c, P.d_ctr, sizeof(c), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         const double est = (double)c.pairs * ((double)U / (double)ns);
+        P.estimated_pairs = (uint64_t)est;
         P.capacity = (uint64_t)(est * 1.10) + (1u << 20);
         size_t free_b = 0, total_b = 0;
         HIP_TRY(hipMemGetInfo(&free_b, &total_b));
@@ -502,82 +529,133 @@ hegj_status make_plan(hegj_index *h, const hegj_join_opts &opts, uint32_t qb, co
     return HEGJ_OK;
 }
 
-// Batches: each launch covers the pending units; a unit whose neighbours do
-// not fit in the batch's buffer is deferred to the next batch.
-hegj_status run_batches(hegj_index *h, const hegj_join_opts &opts, uint32_t qb, JoinPlan &P,
-                        hegj_result *r) {
-    hipStream_t s = h->stream;
-    const uint32_t n = h->g.n, U = P.U;
-    r->info.n_units = U;
-    r->info.estimate_ms = P.estimate_ms;
-    HIP_TRY(dev_malloc_n(&r->d_row_off, n));
-    HIP_TRY(dev_malloc_n(&r->d_row_cnt, n));
-    HIP_TRY(hipMemsetAsync(r->d_row_off, 0, sizeof(uint64_t) * n, s));
-    HIP_TRY(hipMemsetAsync(r->d_row_cnt, 0, sizeof(uint32_t) * n, s));
-    JoinArgs a = base_args(h, opts, P);
-    a.row_off = r->d_row_off;
-    a.row_cnt = r->d_row_cnt;
-    HIP_TRY(hipMemsetAsync(P.d_ctr, 0, sizeof(Counters), s));
-    const uint2 *cur = P.units;
-    uint32_t cur_n = U;
-    uint64_t base = 0, capacity = P.capacity;
+// Runs lists of work units into a result's batches.  A unit whose rows do
+// not fit in the open batch is deferred; deferred units go to a new batch,
+// sized exactly by a count-only pass over them.  A batch may stay open across
+// run() calls (the hybrid modes feed the GPU its share in chunks): its pair
+// hash is taken when it closes, once every row it holds is written.  With
+// `last` the batch closes right after the launch (no further units will
+// come, or the launch defers and the batch is full anyway), so the hash pass
+// follows the join without a host round trip.
+struct BatchRunner {
+    hegj_index *h;
+    const hegj_join_opts &opts;
+    uint32_t qb;
+    JoinPlan &P;
+    hegj_result *r;
+    JoinArgs a{};
+    uint64_t base = 0, capacity = 0, used = 0;
+    bool open = false;
     double join_ms = 0;
     uint32_t launches = 0;
     Counters c{};
-    while (cur_n > 0) {
+
+    BatchRunner(hegj_index *h_, const hegj_join_opts &o, uint32_t qb_, JoinPlan &P_, hegj_result *r_)
+        : h(h_), opts(o), qb(qb_), P(P_), r(r_) {}
+
+    hegj_status init() {
+        hipStream_t s = h->stream;
+        const uint32_t n = h->g.n;
+        r->info.n_units = P.U;
+        r->info.estimate_ms = P.estimate_ms;
+        r->info.estimated_pairs = P.estimated_pairs;
+        HIP_TRY(dev_malloc_n(&r->d_row_off, n));
+        HIP_TRY(dev_malloc_n(&r->d_row_cnt, n));
+        HIP_TRY(hipMemsetAsync(r->d_row_off, 0, sizeof(uint64_t) * n, s));
+        HIP_TRY(hipMemsetAsync(r->d_row_cnt, 0, sizeof(uint32_t) * n, s));
+        a = base_args(h, opts, P);
+        a.row_off = r->d_row_off;
+        a.row_cnt = r->d_row_cnt;
+        HIP_TRY(hipMemsetAsync(P.d_ctr, 0, sizeof(Counters), s));
+        capacity = P.capacity;
+        return HEGJ_OK;
+    }
+
+    hegj_status open_batch() {
+        if (r->batches.size() >= (size_t)kMaxBatches)
+            return fail(HEGJ_ERR_CAPACITY, "result needs more than 64 batches");
         uint32_t *buf = nullptr;
         HIP_TRY(dev_malloc_n(&buf, capacity));
         r->batches.push_back(buf);
         r->bases.push_back(base);
-        // reset per-batch counters (hash / evals / pairs accumulate)
-        HIP_TRY(hipMemsetAsync(P.d_ctr, 0, offsetof(Counters, hash), s));
-        a.units = cur;
-        a.n_units = cur_n;
-        a.units_per_grab = opts.units_per_grab ? opts.units_per_grab : grab_for(P, cur_n);
-        if (cur == P.units && cur_n == U) {
-            set_parts(a, P.parts, P.part_end);
-        } else {
-            uint32_t one_end = cur_n;
-            set_parts(a, 1, &one_end);
-        }
+        // the batch cursor starts at zero (hash / evals / pairs accumulate)
+        HIP_TRY(hipMemsetAsync(&P.d_ctr->cursor, 0, sizeof(P.d_ctr->cursor), h->stream));
         a.out = buf;
         a.out_base = base;
         a.capacity = capacity;
-        a.count_only = 0;
-        {
-            EventTimer t(s);
-            HEGJ_TRY(launch_join(a, qb, blocks_for(P, cur_n, a.units_per_grab), s));
-            join_ms += t.stop();
-            launches++;
-        }
-        HEGJ_TRY(launch_pair_hash(a, capacity, s));
-        HIP_TRY(hipMemcpyAsync(&c, P.d_ctr, sizeof(c), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        if (c.mismatch)
-            return fail(HEGJ_ERR_HIP, "join: the write pass of " + std::to_string(c.mismatch) +
-                                          " work units disagreed with their counts");
-        if (r->batches.size() >= (size_t)kMaxBatches && c.n_deferred)
-            return fail(HEGJ_ERR_CAPACITY, "result needs more than 64 batches");
+        open = true;
+        used = 0;
+        return HEGJ_OK;
+    }
+
+    hegj_status close_batch(bool hashed) {
+        if (!hashed) HEGJ_TRY(launch_pair_hash(a, capacity, h->stream));
         base += capacity;
-        if (c.n_deferred == cur_n) {
-            // nothing fit: one unit alone exceeds the batch; grow it
-            dev_free(r->batches.back());
-            r->batches.pop_back();
-            r->bases.pop_back();
-            base -= capacity;
-            capacity *= 2;
-            continue;
-        }
-        r->info.n_deferred += c.n_deferred;
-        if (c.n_deferred) {
-            // deferred[] indexes the current list; gather into the other one
-            uint2 *nxt = (cur == P.units) ? P.units_next : P.units;
+        open = false;
+        return HEGJ_OK;
+    }
+
+    hegj_status run(const uint2 *units, uint32_t nu, bool parts, bool last) {
+        hipStream_t s = h->stream;
+        const uint2 *cur = units;
+        uint32_t cur_n = nu;
+        while (cur_n > 0) {
+            if (!open) HEGJ_TRY(open_batch());
+            const uint64_t before = used;
+            // queue heads and the deferral count restart; the cursor continues
+            HIP_TRY(hipMemsetAsync(P.d_ctr, 0, offsetof(Counters, cursor), s));
+            a.units = cur;
+            a.n_units = cur_n;
+            a.units_per_grab = opts.units_per_grab ? opts.units_per_grab : grab_for(P, cur_n);
+            if (parts && cur == units && cur_n == P.U) {
+                set_parts(a, P.parts, P.part_end);
+            } else {
+                uint32_t one_end = cur_n;
+                set_parts(a, 1, &one_end);
+            }
+            a.count_only = 0;
+            {
+                EventTimer t(s);
+                HEGJ_TRY(launch_join(a, qb, blocks_for(P, cur_n, a.units_per_grab), s));
+                join_ms += t.stop();
+                launches++;
+            }
+            if (last) HEGJ_TRY(launch_pair_hash(a, capacity, s));
+            HIP_TRY(hipMemcpyAsync(&c, P.d_ctr, sizeof(c), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            if (c.mismatch)
+                return fail(HEGJ_ERR_HIP, "join: the write pass of " + std::to_string(c.mismatch) +
+                                              " work units disagreed with their counts");
+            used = c.cursor;
+            if (c.n_deferred == 0) {
+                if (last) HEGJ_TRY(close_batch(true));
+                break;
+            }
+            if (c.n_deferred == cur_n && before == 0) {
+                // nothing fit in an empty batch: one unit alone exceeds it; grow
+                dev_free(r->batches.back());
+                r->batches.pop_back();
+                r->bases.pop_back();
+                open = false;
+                capacity *= 2;
+                continue;
+            }
+            HEGJ_TRY(close_batch(last));  // full
+            r->info.n_deferred += c.n_deferred;
+            // deferred[] indexes the current list; gather into a buffer that
+            // is neither the current list nor the caller's
+            uint2 *nxt = (cur == P.units_next) ? P.units_tmp : P.units_next;
+            if (!nxt) {
+                HIP_TRY(dev_malloc_n(&P.units_tmp, (size_t)P.U + 1));
+                nxt = P.units_tmp;
+            }
             HEGJ_TRY(gather_units(cur, P.deferred, c.n_deferred, nxt, s));
             cur = nxt;
+            cur_n = c.n_deferred;
             // size the next batch exactly: count-only pass over what is left
             JoinArgs e = a;
             e.units = cur;
-            e.n_units = c.n_deferred;
+            e.n_units = cur_n;
             e.units_per_grab = grab_for(P, e.n_units);
             uint32_t one_end = e.n_units;
             set_parts(e, 1, &one_end);
@@ -596,17 +674,247 @@ hegj_status run_batches(hegj_index *h, const hegj_join_opts &opts, uint32_t qb, 
             HIP_TRY(hipStreamSynchronize(s));
             if (!opts.result_capacity) capacity = std::max<uint64_t>(c2.pairs, 1);
         }
-        cur_n = c.n_deferred;
+        return HEGJ_OK;
     }
-    r->info.total_neighbors = c.pairs;
-    r->info.pair_hash = c.hash;
-    r->info.evals_executed = c.evals;
-    r->info.band_rechecks = c.band;
-    r->info.refine_used = P.f32 ? HEGJ_REFINE_F32 : HEGJ_REFINE_FP64;
+
+    // Closes the open batch and fills the result's figures.
+    hegj_status finish() {
+        hipStream_t s = h->stream;
+        if (open) {
+            HEGJ_TRY(close_batch(false));
+            HIP_TRY(hipMemcpyAsync(&c, P.d_ctr, sizeof(c), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+        }
+        r->info.total_neighbors = c.pairs;
+        r->info.pair_hash = c.hash;
+        r->info.evals_executed = c.evals;
+        r->info.band_rechecks = c.band;
+        r->info.refine_used = P.f32 ? HEGJ_REFINE_F32 : HEGJ_REFINE_FP64;
+        r->info.n_batches = (uint32_t)r->batches.size();
+        r->info.join_ms = join_ms;
+        r->info.join_launches = launches;
+        r->info.join_kernel_avg_ms = launches ? join_ms / launches : 0.0;
+        return HEGJ_OK;
+    }
+};
+
+// Algorithm 0: every unit in one run (per-XCD ranges on the first launch).
+hegj_status run_batches(hegj_index *h, const hegj_join_opts &opts, uint32_t qb, JoinPlan &P,
+                        hegj_result *r) {
+    BatchRunner B(h, opts, qb, P, r);
+    HEGJ_TRY(B.init());
+    HEGJ_TRY(B.run(P.units, P.U, true, true));
+    return B.finish();
+}
+
+// ---------------------------------------------------------------------------
+// Hybrid modes (algorithms 1-3).  The work queue is the unit list in workload
+// order; `head` is the GPU's next unit, `tail` one past the CPU's last free
+// unit, both under one host lock (WorkQueue.cpp:31-118 keeps queueIndex /
+// queueIndexCPU the same way).  The GPU side runs its chunks through the
+// batch runner on the calling thread; the CPU threads first build the
+// Super-EGO order, then take batches of whole units from the tail.
+// ---------------------------------------------------------------------------
+double g_model_gpu[4] = {-0.9589, 0.4342, 0.3532, 0.2686};   // StaticPartition.cpp:42-46
+double g_model_cpu[4] = {-7.2411, 0.9499, 0.7669, 0.4269};   // StaticPartition.cpp:55-59
+std::mutex g_model_mu;
+
+struct CpuRows {  // rows one CPU thread answered
+    std::vector<uint32_t> qid, cnt, nbrs;
+    uint64_t scanned = 0;
+};
+
+double seconds_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+hegj_status selfjoin_hybrid(hegj_index *h, const hegj_join_opts &opts, uint32_t qb,
+                            const hegj_hybrid_opts &ho, hegj_result *r) {
+    const auto t0 = std::chrono::steady_clock::now();
+    hipStream_t s = h->stream;
+    const uint32_t n = h->g.n, dim = h->g.dim, C = h->ix.C;
+    hegj_hybrid_info &hi = r->hinfo;
+    hi.mode = ho.mode;
+    hi.cpu_threads = ho.cpu_threads ? ho.cpu_threads : std::max(1u, std::thread::hardware_concurrency());
+    const uint32_t cpu_batch = ho.cpu_batch ? ho.cpu_batch : 1024u;
+    // the queue: every cell's units in workload order
+    JoinPlan P;
+    HEGJ_TRY(make_plan(h, opts, qb, h->d_order, P, true));
+    const uint32_t U = P.U;
+    std::vector<uint2> units(U);
+    std::vector<uint32_t> cstart(C + 1), ila(n);
+    std::vector<double> pts((size_t)n * dim);
+    HIP_TRY(hipMemcpyAsync(units.data(), P.units, sizeof(uint2) * U, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(cstart.data(), h->ix.cell_start, sizeof(uint32_t) * (C + 1), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(ila.data(), h->ix.ila, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(pts.data(), h->ix.pts, sizeof(double) * pts.size(), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    // queries before each unit
+    std::vector<uint64_t> qpre(U + 1, 0);
+    for (uint32_t u = 0; u < U; u++) {
+        const uint32_t cell = units[u].x, size = cstart[cell + 1] - cstart[cell];
+        qpre[u + 1] = qpre[u] + std::min(qb, size - units[u].y);
+    }
+    // GPU share: everything (dynamic), the static split, or nothing
+    uint32_t gpu_limit = U;
+    if (ho.mode == HEGJ_MODE_CPU) {
+        gpu_limit = 0;
+    } else if (ho.mode == HEGJ_MODE_STATIC) {
+        double frac = ho.gpu_fraction;
+        if (!(frac > 0.0)) frac = hegj_static_partition_model(n, dim, h->g.eps);
+        hi.gpu_fraction = frac;
+        // partitionedDBSIZE = DBSIZE * staticPartition in float (GPU.cu:437),
+        // moved up to the next unit boundary
+        const uint64_t target = (uint64_t)((float)n * (float)frac);
+        gpu_limit = (uint32_t)(std::lower_bound(qpre.begin(), qpre.end(), target) - qpre.begin());
+        gpu_limit = std::min(gpu_limit, U);
+    }
+    const bool dynamic = ho.mode == HEGJ_MODE_HYBRID;
+    std::mutex mu;
+    uint32_t head = 0, tail = U;
+
+    // CPU side
+    std::vector<CpuRows> rows(hi.cpu_threads);
+    EgoIndex *ego = nullptr;
+    hegj_status cpu_status = HEGJ_OK;
+    std::string cpu_error;
+    std::thread cpu([&]() {
+        double ro = 0, so = 0;
+        const hegj_status st = ego_build(pts.data(), n, dim, h->g.eps, hi.cpu_threads, &ego, &ro, &so);
+        if (st != HEGJ_OK) {
+            cpu_status = st;
+            cpu_error = hegj_last_error();
+            return;
+        }
+        hi.reorder_s = ro;
+        hi.sort_s = so;
+        const auto tc = std::chrono::steady_clock::now();
+        std::vector<std::thread> workers;
+        for (uint32_t t = 0; t < hi.cpu_threads; t++)
+            workers.emplace_back([&, t]() {
+                CpuRows &mine = rows[t];
+                std::vector<uint32_t> row;
+                while (true) {
+                    uint32_t a, b;
+                    {
+                        std::lock_guard<std::mutex> lk(mu);
+                        const uint32_t floor_u = dynamic ? head : gpu_limit;
+                        if (tail <= floor_u) break;
+                        b = tail;
+                        a = b;
+                        while (a > floor_u && qpre[b] - qpre[a] < cpu_batch) a--;
+                        tail = a;
+                    }
+                    for (uint32_t u = a; u < b; u++) {
+                        const uint32_t p0 = cstart[units[u].x] + units[u].y;
+                        const uint32_t nq = (uint32_t)(qpre[u + 1] - qpre[u]);
+                        for (uint32_t i = 0; i < nq; i++) {
+                            const uint32_t q = ila[p0 + i];
+                            row.clear();
+                            mine.scanned += ego_query(*ego, q, row);
+                            mine.qid.push_back(q);
+                            mine.cnt.push_back((uint32_t)row.size());
+                            mine.nbrs.insert(mine.nbrs.end(), row.begin(), row.end());
+                        }
+                    }
+                }
+            });
+        for (auto &w : workers) w.join();
+        hi.cpu_s = seconds_since(tc);
+        hi.cpu_end_s = seconds_since(t0);
+    });
+
+    // GPU side, on this thread
+    BatchRunner B(h, opts, qb, P, r);
+    hegj_status gst = B.init();
+    const auto tg = std::chrono::steady_clock::now();
+    const uint32_t chunk = ho.gpu_chunk ? ho.gpu_chunk : std::max<uint32_t>(1, (U + 15) / 16);
+    uint64_t gq = 0;
+    while (gst == HEGJ_OK) {
+        uint32_t a, b;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            const uint32_t end = std::min(tail, gpu_limit);
+            if (head >= end) break;
+            a = head;
+            b = std::min(end, head + chunk);
+            head = b;
+        }
+        gst = B.run(P.units + a, b - a, false, false);
+        gq += qpre[b] - qpre[a];
+    }
+    if (gst == HEGJ_OK) gst = B.finish();
+    hi.gpu_s = seconds_since(tg);
+    hi.gpu_end_s = seconds_since(t0);
+    if (gst != HEGJ_OK) {  // stop the CPU side from taking more, then report
+        const std::string msg = hegj_last_error();
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            tail = 0;
+            gpu_limit = 0;
+        }
+        cpu.join();
+        ego_free(ego);
+        return fail(gst, msg);
+    }
+    cpu.join();
+    ego_free(ego);
+    if (cpu_status != HEGJ_OK) return fail(cpu_status, cpu_error);
+    hi.n_queries_gpu = (uint32_t)gq;
+    hi.pairs_gpu = r->info.total_neighbors;
+
+    // the CPU rows join the result as one more batch
+    uint64_t cpu_pairs = 0, cpu_hash = 0, m = 0;
+    for (const CpuRows &c : rows) {
+        cpu_pairs += c.nbrs.size();
+        m += c.qid.size();
+        hi.candidates_cpu += c.scanned;
+    }
+    hi.n_queries_cpu = (uint32_t)m;
+    hi.pairs_cpu = cpu_pairs;
+    if (m > 0) {
+        std::vector<uint32_t> qid, cnt, nb;
+        std::vector<uint64_t> off;
+        qid.reserve(m);
+        cnt.reserve(m);
+        off.reserve(m);
+        nb.reserve(cpu_pairs);
+        for (const CpuRows &c : rows) {
+            size_t k = 0;
+            for (size_t i = 0; i < c.qid.size(); i++) {
+                qid.push_back(c.qid[i]);
+                cnt.push_back(c.cnt[i]);
+                off.push_back(nb.size());
+                for (uint32_t j = 0; j < c.cnt[i]; j++, k++) {
+                    nb.push_back(c.nbrs[k]);
+                    cpu_hash += mix64(((uint64_t)c.qid[i] << 32) | c.nbrs[k]);
+                }
+            }
+        }
+        if (r->batches.size() >= (size_t)kMaxBatches)
+            return fail(HEGJ_ERR_CAPACITY, "result needs more than 64 batches");
+        uint32_t *buf = nullptr;
+        HIP_TRY(dev_malloc_n(&buf, std::max<uint64_t>(cpu_pairs, 1)));
+        r->batches.push_back(buf);
+        r->bases.push_back(B.base);
+        Scratch sc;
+        uint32_t *d_qid, *d_cnt;
+        uint64_t *d_off;
+        HIP_TRY(sc.get(&d_qid, m));
+        HIP_TRY(sc.get(&d_cnt, m));
+        HIP_TRY(sc.get(&d_off, m));
+        if (cpu_pairs)
+            HIP_TRY(hipMemcpyAsync(buf, nb.data(), sizeof(uint32_t) * cpu_pairs, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(d_qid, qid.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(d_cnt, cnt.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(d_off, off.data(), sizeof(uint64_t) * m, hipMemcpyHostToDevice, s));
+        HEGJ_TRY(scatter_rows(d_qid, d_off, d_cnt, (uint32_t)m, B.base, r->d_row_off, r->d_row_cnt, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    r->info.total_neighbors += cpu_pairs;
+    r->info.pair_hash += cpu_hash;
     r->info.n_batches = (uint32_t)r->batches.size();
-    r->info.join_ms = join_ms;
-    r->info.join_launches = launches;
-    r->info.join_kernel_avg_ms = launches ? join_ms / launches : 0.0;
+    r->hybrid = true;
     return HEGJ_OK;
 }
 
@@ -691,6 +999,7 @@ hegj_status selfjoin_graph(hegj_index *h, const hegj_join_opts &opts, uint32_t q
     GraphPlan &G = *h->gplan;
     r->info.n_units = G.P.U;
     r->info.estimate_ms = G.P.estimate_ms;
+    r->info.estimated_pairs = G.P.estimated_pairs;
     HIP_TRY(dev_malloc_n(&r->d_row_off, n));
     HIP_TRY(dev_malloc_n(&r->d_row_cnt, n));
     HIP_TRY(hipMemsetAsync(r->d_row_off, 0, sizeof(uint64_t) * n, s));
@@ -788,6 +1097,7 @@ hegj_status selfjoin_mfma(hegj_index *h, const uint32_t *order, hegj_result *r) 
     HIP_TRY(hipMemcpyAsync(&hash, d_hash, sizeof(hash), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     r->info.total_neighbors = o.pairs;
+    r->info.estimated_pairs = o.pairs;
     r->info.pair_hash = hash;
     r->info.evals_executed = o.evals;
     r->info.band_rechecks = o.band;
@@ -921,7 +1231,7 @@ hegj_status hegj_index_get_info(const hegj_index *h, hegj_index_info *info) {
     info->n = h->g.n;
     info->dim = h->g.dim;
     info->indexed_dims = h->g.k;
-    info->dtype = HEGJ_DTYPE_F64;
+    info->dtype = h->g.f32dt ? HEGJ_DTYPE_F32 : HEGJ_DTYPE_F64;
     info->nNonEmptyCells = h->ix.C;
     info->n_rows = h->ix.R;
     info->n_cell0_queries = h->n_dead;
@@ -1072,13 +1382,19 @@ hegj_status hegj_selfjoin(hegj_index *h, const hegj_join_opts *opts_in, hegj_res
     auto t_start = std::chrono::steady_clock::now();
     HEGJ_TRY(set_device(h->device));
     hegj_join_opts opts;
-    if (opts_in)
+    if (opts_in) {
         opts = *opts_in;
-    else
+    } else {
         hegj_join_opts_tuned(&opts, h->g.dim);
+        // the table is tuned for DTYPE=double: float data take the grid kernels
+        if (h->g.f32dt && opts.refine == HEGJ_REFINE_MFMA) opts.refine = HEGJ_REFINE_AUTO;
+    }
     if (opts.refine > HEGJ_REFINE_F32) return fail(HEGJ_ERR_INVALID_ARG, "unknown refine");
+    if (opts.refine == HEGJ_REFINE_MFMA && h->g.f32dt)
+        return fail(HEGJ_ERR_UNSUPPORTED, "the MFMA refine is built for DTYPE=double");
     const bool mfma = opts.refine == HEGJ_REFINE_MFMA ||
-                      (opts.refine == HEGJ_REFINE_AUTO && h->g.dim >= 16 && gb_kblocks(h->g.dim) != 0);
+                      (opts.refine == HEGJ_REFINE_AUTO && !h->g.f32dt && h->g.dim >= 16 &&
+                       gb_kblocks(h->g.dim) != 0);
     if (mfma && gb_kblocks(h->g.dim) == 0)
         return fail(HEGJ_ERR_UNSUPPORTED, "no MFMA refine for this dimensionality");
     const uint32_t qb = opts.queries_per_unit ? opts.queries_per_unit : default_qb(h->g.dim);
@@ -1112,7 +1428,7 @@ hegj_status hegj_selfjoin(hegj_index *h, const hegj_join_opts *opts_in, hegj_res
         return HEGJ_OK;
     }
     r->info.refine_used = HEGJ_REFINE_FP64;
-    if (opts.use_graph) {
+    if (opts.use_graph && !h->g.f32dt) {
         bool fits = false;
         HEGJ_TRY(selfjoin_graph(h, opts, qb, order, r.get(), &fits));
         if (!fits) {  // the cached capacity was too small: this call runs eagerly
@@ -1136,6 +1452,74 @@ hegj_status hegj_selfjoin(hegj_index *h, const hegj_join_opts *opts_in, hegj_res
     HEGJ_TRY(mark_done(r.get(), s));
     *out = r.release();
     return HEGJ_OK;
+}
+
+void hegj_hybrid_opts_default(hegj_hybrid_opts *o, uint32_t mode) {
+    std::memset(o, 0, sizeof(*o));
+    o->mode = mode;
+}
+
+hegj_status hegj_selfjoin_hybrid(hegj_index *h, const hegj_join_opts *opts_in,
+                                 const hegj_hybrid_opts *ho, hegj_result **out) {
+    if (!h || !ho || !out) return fail(HEGJ_ERR_INVALID_ARG, "NULL argument");
+    if (ho->mode == HEGJ_MODE_GPU) return hegj_selfjoin(h, opts_in, out);
+    if (ho->mode > HEGJ_MODE_CPU) return fail(HEGJ_ERR_INVALID_ARG, "unknown search mode");
+    if (ho->mode == HEGJ_MODE_STATIC && ho->gpu_fraction >= 1.0)
+        return fail(HEGJ_ERR_INVALID_ARG, "the static partition must be in ]0.0, 1.0[");
+    const auto t_start = std::chrono::steady_clock::now();
+    HEGJ_TRY(set_device(h->device));
+    hegj_join_opts opts;
+    if (opts_in) {
+        opts = *opts_in;
+    } else {
+        hegj_join_opts_tuned(&opts, h->g.dim);
+        if (opts.refine == HEGJ_REFINE_MFMA) opts.refine = HEGJ_REFINE_AUTO;  // the grid kernels
+    }
+    if (opts.refine == HEGJ_REFINE_MFMA)
+        return fail(HEGJ_ERR_UNSUPPORTED, "the hybrid modes run the grid join kernels, not the MFMA refine");
+    if (opts.refine > HEGJ_REFINE_F32) return fail(HEGJ_ERR_INVALID_ARG, "unknown refine");
+    const uint32_t qb = opts.queries_per_unit ? opts.queries_per_unit : default_qb(h->g.dim);
+    if (qb != 4 && qb != 8 && qb != 16) return fail(HEGJ_ERR_INVALID_ARG, "queries_per_unit must be 4, 8 or 16");
+    std::unique_ptr<hegj_result> r(new hegj_result());
+    r->idx = h;
+    HEGJ_TRY(ensure_workload(h));
+    r->info.total_candidates = h->total_candidates;
+    r->info.workload_ms = h->workload_ms;
+    r->info.n_queries = h->g.n - h->n_dead;
+    HEGJ_TRY(selfjoin_hybrid(h, opts, qb, *ho, r.get()));
+    r->info.total_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    HEGJ_TRY(mark_done(r.get(), h->stream));
+    *out = r.release();
+    return HEGJ_OK;
+}
+
+hegj_status hegj_result_get_hybrid_info(const hegj_result *r, hegj_hybrid_info *info) {
+    if (!r || !info) return fail(HEGJ_ERR_INVALID_ARG, "NULL argument");
+    if (!r->hybrid) return fail(HEGJ_ERR_INVALID_ARG, "not a hybrid result");
+    *info = r->hinfo;
+    return HEGJ_OK;
+}
+
+double hegj_static_partition_model(uint32_t n, uint32_t dim, double epsilon) {
+    double g[4], c[4];
+    {
+        std::lock_guard<std::mutex> lk(g_model_mu);
+        std::memcpy(g, g_model_gpu, sizeof(g));
+        std::memcpy(c, g_model_cpu, sizeof(c));
+    }
+    const double ld = std::log((double)dim), ln = std::log((double)n), le = std::log(epsilon);
+    const double tg = std::exp(g[0] + g[1] * ld + g[2] * ln + g[3] * le);
+    const double tc = std::exp(c[0] + c[1] * ld + c[2] * ln + c[3] * le);
+    // int gpu_qps = DBSIZE / gpuTimeModel (main.cu:152-154)
+    const double gq = std::trunc((double)n / tg), cq = std::trunc((double)n / tc);
+    return gq + cq > 0 ? gq / (gq + cq) : 1.0;
+}
+
+void hegj_static_partition_set_model(const double *gpu, const double *cpu) {
+    std::lock_guard<std::mutex> lk(g_model_mu);
+    if (gpu) std::memcpy(g_model_gpu, gpu, sizeof(g_model_gpu));
+    if (cpu) std::memcpy(g_model_cpu, cpu, sizeof(g_model_cpu));
 }
 
 hegj_status hegj_f32_thresholds(double epsilon, uint32_t dim, double radius, float *tacc,

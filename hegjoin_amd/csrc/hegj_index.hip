@@ -58,7 +58,10 @@ __global__ __launch_bounds__(kT) void k_cell_ids(const double *__restrict__ pts,
     if (i < g.n) {
         uint64_t key = 0, mult = 1;
         for (uint32_t j = 0; j < g.k; j++) {
-            uint32_t c = (uint32_t)((pts[(uint64_t)i * g.dim + j] - g.minArr[j]) / g.eps);
+            const double x = pts[(uint64_t)i * g.dim + j];
+            // DTYPE=float: the same expression in FP32 (its operands hold floats)
+            uint32_t c = g.f32dt ? (uint32_t)(((float)x - (float)g.minArr[j]) / (float)g.eps)
+                                 : (uint32_t)((x - g.minArr[j]) / g.eps);
             dead |= (c == 0);
             key += (uint64_t)c * mult;
             mult *= g.nCells[j];
@@ -118,6 +121,13 @@ __global__ __launch_bounds__(kT) void k_hash_insert(const uint64_t *__restrict__
     }
 }
 
+// DTYPE=float: the database rounded to float in place (main.cu:120-128 copies
+// the parsed values into a DTYPE array).
+__global__ __launch_bounds__(kT) void k_round_f32(double *__restrict__ pts, uint64_t total) {
+    const uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x;
+    if (i < total) pts[i] = (double)(float)pts[i];
+}
+
 // soa[l*n + p] = pts[ila[p]*dim + l]: coordinates in cell order, SoA.
 __global__ __launch_bounds__(kT) void k_gather_soa(const double *__restrict__ pts,
                                                    const uint32_t *__restrict__ ila, uint32_t n,
@@ -144,16 +154,24 @@ __global__ __launch_bounds__(kT) void k_gather_aos(const double *__restrict__ pt
 // false when the grid cannot be represented: a dimension with 2^32 cells or
 // more, or more than 2^64 - 1 cells in total (the reference's uint32 cast and
 // uint64 product would wrap there and cell ids would collide).
+// With DTYPE=float every step is the same expression in FP32.
 bool finish_dims(const double *mn, const double *mx, uint32_t k, double eps, GridGeom *g,
                  double *maxArr, uint64_t *totalCells) {
+    const float epsf = (float)eps;
     for (uint32_t j = 0; j < k; j++) {
-        g->minArr[j] = mn[j] - eps;
-        maxArr[j] = mx[j] + eps;
+        if (g->f32dt) {
+            g->minArr[j] = (double)((float)mn[j] - epsf);
+            maxArr[j] = (double)((float)mx[j] + epsf);
+        } else {
+            g->minArr[j] = mn[j] - eps;
+            maxArr[j] = mx[j] + eps;
+        }
     }
     unsigned __int128 t = 1;
     bool ok = true;
     for (uint32_t j = 0; j < k; j++) {
-        const double c = std::ceil((maxArr[j] - g->minArr[j]) / eps);
+        const double c = g->f32dt ? (double)std::ceil(((float)maxArr[j] - (float)g->minArr[j]) / epsf)
+                                  : std::ceil((maxArr[j] - g->minArr[j]) / eps);
         ok &= c < 4294967296.0;
         g->nCells[j] = ok ? (uint32_t)c : 1u;
         t *= g->nCells[j];
@@ -168,6 +186,11 @@ hegj_status build_index(GridGeom *g, double *maxArr, uint64_t *totalCells, DevIn
                         uint32_t *n_dead, hipStream_t s) {
     const uint32_t n = g->n, k = g->k, dim = g->dim;
     Scratch sc;
+    if (g->f32dt) {
+        const uint64_t total = (uint64_t)n * dim;
+        k_round_f32<<<ceil_div(total, kT), kT, 0, s>>>(ix->pts, total);
+        HIP_TRY(hipGetLastError());
+    }
     // -- S1: extents ---------------------------------------------------------
     uint32_t mm_blocks = std::min<uint32_t>(ceil_div(n, kT), 1024);
     double *partials;

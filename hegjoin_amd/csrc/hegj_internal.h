@@ -50,6 +50,8 @@ struct GridGeom {
     uint32_t dim;
     uint32_t n;
     uint32_t emulate_cell0;
+    uint32_t f32dt;     // DTYPE=float (params.h:24): minArr, eps and thresh hold float
+                        // values; cell ids and distances are computed in FP32
 };
 
 // Device-resident grid index.  Naming follows the reference: a "cell" is a
@@ -110,6 +112,8 @@ void cache_release_device(int dev);  // hipFree the cached buffers of one device
 
 // Largest s with RN(sqrt(s)) <= eps (the distance predicate without sqrt).
 double sqrt_threshold(double eps);
+// The same for DTYPE=float: largest float s with RN32(sqrt(s)) <= eps.
+float sqrt_threshold_f(float eps);
 
 // ---- primitives (hegj_prims.hip) -----------------------------------------
 // Exclusive scan; writes the total to *d_total when non-null.
@@ -169,8 +173,10 @@ void free_index(DevIndex *ix);
 
 // ---- join (hegj_join.hip) ------------------------------------------------
 hegj_status launch_workload(const GridGeom &g, const DevIndex &ix, uint32_t *work, hipStream_t s);
+// all_cells: also the cells the cell-0 wrap silences (their queries get
+// empty rows), so the units cover every point: the hybrid modes' queue.
 hegj_status build_units(const GridGeom &g, const DevIndex &ix, uint32_t qb, const uint32_t *order,
-                        uint2 **units, uint32_t *n_units, hipStream_t s);
+                        uint2 **units, uint32_t *n_units, hipStream_t s, bool all_cells = false);
 // dev_args != nullptr: the kernel reads its arguments from there (graph replay;
 // `a` then only selects the instantiation by a.g.dim).
 hegj_status launch_join(const JoinArgs &a, uint32_t qb, uint32_t blocks, hipStream_t s,
@@ -195,7 +201,8 @@ bool f32_supported(uint32_t dim, uint32_t qb);
 // Zero `bytes` (a multiple of 4) with a kernel (safe inside graph capture).
 hegj_status zero_words(void *p, size_t bytes, hipStream_t s);
 // Workgroups of the persistent join kernel that fit on the device at once.
-hegj_status join_occupancy(uint32_t dim, uint32_t qb, int device, uint32_t *blocks);
+// flt: the DTYPE=float generic instance.
+hegj_status join_occupancy(uint32_t dim, uint32_t qb, int device, uint32_t *blocks, bool flt = false);
 
 // ---- MFMA guard-band refine (hegj_mfma.hip) --------------------------------
 struct GbOutput {
@@ -293,9 +300,23 @@ hegj_status gather_rows(const std::vector<uint32_t *> &batches, const std::vecto
                         uint32_t n, uint32_t *out, hipStream_t s);
 
 hegj_status stream_copy(uint64_t bytes, int reps, double *ms_per_copy);
+// row_off[qid[i]] = base + off[i], row_cnt[qid[i]] = cnt[i] (rows answered
+// on the host, added to a device result).
+hegj_status scatter_rows(const uint32_t *qid, const uint64_t *off, const uint32_t *cnt, uint32_t m,
+                         uint64_t base, uint64_t *row_off, uint32_t *row_cnt, hipStream_t s);
+
+// ---- Super-EGO on the host (hegj_ego.cpp) ---------------------------------
+struct EgoIndex;
+// Reorders the dimensions and EGO-sorts the points (row-major n x dim).
+hegj_status ego_build(const double *pts, uint32_t n, uint32_t dim, double eps, unsigned threads,
+                      EgoIndex **out, double *reorder_s, double *sort_s);
+void ego_free(EgoIndex *e);
+// Appends the neighbours (original ids) of original point q to row; returns
+// the candidates scanned.
+uint64_t ego_query(const EgoIndex &e, uint32_t q, std::vector<uint32_t> &row);
 
 // ---- device helpers --------------------------------------------------------
-__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
     uint64_t z = x + 0x9E3779B97F4A7C15ull;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;

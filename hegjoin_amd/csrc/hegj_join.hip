@@ -93,12 +93,17 @@ __device__ uint32_t build_rows(const JoinArgs &a, uint32_t cell, RowList &rl, ui
         const double x = ix.soa[(uint64_t)j * g.n + qpos0 + qi];
         double f = (x - g.minArr[j]) / g.eps - (double)cj;
         f = f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
-        const double mg = 1e-6 + 2e-15 * ((double)cj + 2.0);
+        // DTYPE=float assigns cells in FP32: two roundings of relative 2^-24
+        // move a point by at most 1.2e-7 (c_j + 2) cells
+        const double mg = g.f32dt ? 1e-6 + 3e-7 * ((double)cj + 2.0) : 1e-6 + 2e-15 * ((double)cj + 2.0);
         const double gm = f - mg > 0.0 ? f - mg : 0.0, gp = 1.0 - f - mg > 0.0 ? 1.0 - f - mg : 0.0;
         rl.gap[qi * kMaxK + j] = make_float2((float)(gm * gm), (float)(gp * gp));
     }
     wave_lds_sync();
-    const float thr = a.prune ? (float)(g.thresh / (g.eps * g.eps)) * (1.0f + 1e-5f) : INFINITY;
+    // DTYPE=float: the FP32 chain accepts squared distances up to about
+    // (1 + (dim + 4) 2^-23) T
+    const float rel = 1e-5f + (g.f32dt ? 1.2e-7f * (float)(g.dim + 4) : 0.0f);
+    const float thr = a.prune ? (float)(g.thresh / (g.eps * g.eps)) * (1.0f + rel) : INFINITY;
     const uint32_t nrows = pow3(k - 1);
     // Lane l takes rows l, l + 64, l + 128, l + 192: each dependent step
     // below (hash probe, row bounds, dim-0 window, cell bounds) issues the
@@ -393,13 +398,45 @@ __device__ __forceinline__ void dist_regs_lds(const double *x, const double *qs,
 // Generic dimension count (runtime or large D): candidate coordinates are
 // streamed in tiles of 16 dims, acc keeps the dimension order.  The unit's
 // query coordinates come from LDS (qs[qi * dim + l]).
-template <int D, int QB>
+// FLT: DTYPE=float (params.h:24).  The coordinates are floats held in
+// doubles; the chain is the reference's FP32 one, d = fl32(c - q),
+// acc = fmaf(d, d, acc), returned widened to double (exact).
+template <int D, int QB, bool FLT = false>
 __device__ __forceinline__ void dist_stream(const JoinArgs &a, uint32_t pos, bool valid,
                                             const double *qs, uint32_t nq, double *acc) {
     const uint32_t dim = D > 0 ? (uint32_t)D : a.g.dim;
     const uint64_t n = a.g.n;
     const double *__restrict__ soa = a.ix.soa;
     constexpr int DT = 16;
+    if constexpr (FLT) {
+        float accf[QB];
+#pragma unroll
+        for (int qi = 0; qi < QB; qi++) accf[qi] = 0.0f;
+        for (uint32_t d0 = 0; d0 < dim; d0 += DT) {
+            float x[DT];
+#pragma unroll
+            for (int i = 0; i < DT; i++) {
+                uint32_t l = d0 + i;
+                x[i] = (valid && l < dim) ? (float)soa[(uint64_t)l * n + pos] : 0.0f;
+            }
+#pragma unroll
+            for (int qi = 0; qi < QB; qi++) {
+                if ((uint32_t)qi < nq) {
+#pragma unroll
+                    for (int i = 0; i < DT; i++) {
+                        uint32_t l = d0 + i;
+                        if (l < dim) {
+                            const float dd = x[i] - (float)qs[qi * dim + l];
+                            accf[qi] = __builtin_fmaf(dd, dd, accf[qi]);
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int qi = 0; qi < QB; qi++) acc[qi] = (double)accf[qi];
+        return;
+    }
 #pragma unroll
     for (int qi = 0; qi < QB; qi++) acc[qi] = 0.0;
     for (uint32_t d0 = 0; d0 < dim; d0 += DT) {
@@ -488,11 +525,17 @@ constexpr int join_min_waves() {
 // F32: candidates and queries come from the centred FP32 copy; a pair is
 // accepted when acc32 <= tacc32, rejected when acc32 > trej32, and rechecked
 // with the exact FP64 chain in between (the thresholds bound |acc32 - acc|).
-template <int D, int QB, bool IND = false, bool F32 = false>
+// With DTYPE=float (g.f32dt) the copy is the uncentred float data and
+// tacc32 == trej32 == the float threshold, so acc32 is the reference's own
+// FP32 chain and nothing falls in a band.
+// FLT: DTYPE=float in the generic instance (D == 0): the FP32 chain over the
+// float coordinates held in the FP64 copies.
+template <int D, int QB, bool IND = false, bool F32 = false, bool FLT = false>
 __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())) void k_join(
     JoinArgs av, const JoinArgs *__restrict__ ap) {
     const JoinArgs &a = IND ? *ap : av;
     static_assert(!F32 || (D > 0 && D <= 8 && QB * D <= 48), "FP32 filter: small D only");
+    static_assert(!FLT || (D == 0 && !F32), "DTYPE=float: generic instance only");
     __shared__ RowList rls[kWavesPerBlock];
     __shared__ HitStore hs_all[kWavesPerBlock];
     extern __shared__ double q_dyn[];  // QB * dim query coordinates per wave
@@ -667,7 +710,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())
                                 const uint32_t pos = valid ? cand_pos_adv(rl, nslots, t, slot) : 0u;
                                 uint32_t qoff = (uint32_t)wid * QB * dimq;
                                 __asm__ volatile("" : "+v"(qoff));
-                                dist_stream<D, QB>(a, pos, valid, q_dyn + qoff, nq, acc);
+                                dist_stream<D, QB, FLT>(a, pos, valid, q_dyn + qoff, nq, acc);
                             }
 #pragma unroll
                             for (int qi = 0; qi < QL; qi++)
@@ -848,17 +891,32 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())
                     for (int qi = 0; qi < QB; qi++) {
                         if ((uint32_t)qi >= nq) continue;
                         double acc1 = 0.0;
-                        if constexpr (kRegs) {
+                        if constexpr (F32) {
+                            if (a.g.f32dt) {
+                                // DTYPE=float: the reference's FP32 chain
+                                float af = 0.0f;
+#pragma unroll
+                                for (int l = 0; l < DR; l++) {
+                                    const float dd = (float)x[l] - (float)a.ix.soa[(uint64_t)l * a.g.n + qpos0 + qi];
+                                    af = __builtin_fmaf(dd, dd, af);
+                                }
+                                acc1 = (double)af;
+                            } else {
+#pragma unroll
+                                for (int l = 0; l < DR; l++) {
+                                    const double dd = x[l] - a.ix.soa[(uint64_t)l * a.g.n + qpos0 + qi];
+                                    acc1 = __builtin_fma(dd, dd, acc1);
+                                }
+                            }
+                        } else if constexpr (kRegs) {
 #pragma unroll
                             for (int l = 0; l < DR; l++) {
-                                const double q = F32 ? a.ix.soa[(uint64_t)l * a.g.n + qpos0 + qi]
-                                                 : kQS ? qreg[kQS && !F32 ? qi : 0][kQS && !F32 ? l : 0]
-                                                       : qs[qi * DR + l];
+                                const double q = kQS ? qreg[kQS ? qi : 0][kQS ? l : 0] : qs[qi * DR + l];
                                 const double dd = x[l] - q;
                                 acc1 = __builtin_fma(dd, dd, acc1);
                             }
                         } else {
-                            dist_stream<D, 1>(a, pos, valid, qs + qi * dimq, 1, &acc1);
+                            dist_stream<D, 1, FLT>(a, pos, valid, qs + qi * dimq, 1, &acc1);
                         }
                         const uint64_t m = __ballot(valid && acc1 <= T);
                         if (m == 0) continue;
@@ -905,10 +963,12 @@ __global__ __launch_bounds__(256) void k_cell_workload(GridGeom g, DevIndex ix,
     work[cell] = total;
 }
 
-// Units: cell c contributes ceil(|c| / qb) units when its stencil is live.
+// Units: cell c contributes ceil(|c| / qb) units when its stencil is live
+// (every cell with all_cells: its queries then get empty rows on the GPU).
 __global__ __launch_bounds__(256) void k_unit_counts(GridGeom g, DevIndex ix, uint32_t qb,
                                                      const uint32_t *__restrict__ order,
-                                                     uint32_t *__restrict__ ucount) {
+                                                     uint32_t *__restrict__ ucount,
+                                                     uint32_t all_cells) {
     uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= ix.C) return;
     uint32_t cell = order ? order[i] : i;
@@ -916,7 +976,7 @@ __global__ __launch_bounds__(256) void k_unit_counts(GridGeom g, DevIndex ix, ui
     decode_cell(ix.cell_key[cell], g, c);
     bool live = true;
     for (uint32_t j = 0; j < g.k; j++)
-        if (c[j] == 0 && g.emulate_cell0) live = false;
+        if (c[j] == 0 && g.emulate_cell0 && !all_cells) live = false;
     uint32_t sz = ix.cell_start[cell + 1] - ix.cell_start[cell];
     ucount[i] = live ? (sz + qb - 1) / qb : 0u;
 }
@@ -945,14 +1005,14 @@ hegj_status launch_workload(const GridGeom &g, const DevIndex &ix, uint32_t *wor
 }
 
 hegj_status build_units(const GridGeom &g, const DevIndex &ix, uint32_t qb, const uint32_t *order,
-                        uint2 **units, uint32_t *n_units, hipStream_t s) {
+                        uint2 **units, uint32_t *n_units, hipStream_t s, bool all_cells) {
     Scratch sc;
     uint32_t *ucount, *uoff, *d_total;
     uint32_t C = ix.C;
     HIP_TRY(sc.get(&ucount, C));
     HIP_TRY(sc.get(&uoff, C));
     HIP_TRY(sc.get(&d_total, 1));
-    k_unit_counts<<<ceil_div(C, 256), 256, 0, s>>>(g, ix, qb, order, ucount);
+    k_unit_counts<<<ceil_div(C, 256), 256, 0, s>>>(g, ix, qb, order, ucount, all_cells ? 1u : 0u);
     HIP_TRY(hipGetLastError());
     HEGJ_TRY(scan_u32(ucount, uoff, C, d_total, s));
     uint32_t U;
@@ -995,9 +1055,24 @@ static hegj_status launch_join_t(const JoinArgs &a, const JoinArgs *ap, uint32_t
     return HEGJ_OK;
 }
 
+// DTYPE=float outside the FP32-filter instances: the generic kernel with the
+// FP32 chain (eager launches only).
+template <int QB>
+static hegj_status launch_join_flt(const JoinArgs &a, const JoinArgs *ap, uint32_t blocks,
+                                   hipStream_t s) {
+    if (ap) return fail(HEGJ_ERR_UNSUPPORTED, "DTYPE=float joins are not replayed from a graph");
+    const uint32_t lds = q_lds_bytes<0, QB>(a.g.dim);
+    if (lds > kMaxQueryLds)
+        return fail(HEGJ_ERR_UNSUPPORTED, "queries_per_unit * dim too large for LDS; lower queries_per_unit");
+    k_join<0, QB, false, false, true><<<blocks, 64 * kWavesPerBlock, lds, s>>>(a, nullptr);
+    HIP_TRY(hipGetLastError());
+    return HEGJ_OK;
+}
+
 template <int QB>
 static hegj_status launch_join_qb(const JoinArgs &a, const JoinArgs *ap, uint32_t blocks,
                                   hipStream_t s) {
+    if (a.g.f32dt && !a.f32) return launch_join_flt<QB>(a, ap, blocks, s);
     switch (a.g.dim) {
         case 2: return launch_join_t<2, QB>(a, ap, blocks, s);
         case 6: return launch_join_t<6, QB>(a, ap, blocks, s);
@@ -1087,13 +1162,17 @@ hegj_status zero_words(void *p, size_t bytes, hipStream_t s) {
 
 namespace hegj {
 
-hegj_status join_occupancy(uint32_t dim, uint32_t qb, int device, uint32_t *blocks) {
+hegj_status join_occupancy(uint32_t dim, uint32_t qb, int device, uint32_t *blocks,
+                           bool flt) {
     int per_cu = 0;
     const void *fn = nullptr;
     uint32_t lds = 0;
 #define HEGJ_OCC(D, Q) fn = reinterpret_cast<const void *>(&k_join<D, Q, false>), lds = q_lds_bytes<D, Q>(dim)
     uint32_t d = (dim == 2 || dim == 6 || dim == 18 || dim == 90) ? dim : 0;
-    switch (qb * 1000 + d) {
+    switch (flt ? qb * 1000 + 999 : qb * 1000 + d) {
+        case 4999: fn = reinterpret_cast<const void *>(&k_join<0, 4, false, false, true>), lds = q_lds_bytes<0, 4>(dim); break;
+        case 8999: fn = reinterpret_cast<const void *>(&k_join<0, 8, false, false, true>), lds = q_lds_bytes<0, 8>(dim); break;
+        case 16999: fn = reinterpret_cast<const void *>(&k_join<0, 16, false, false, true>), lds = q_lds_bytes<0, 16>(dim); break;
         case 4002: HEGJ_OCC(2, 4); break;
         case 4006: HEGJ_OCC(6, 4); break;
         case 4018: HEGJ_OCC(18, 4); break;

@@ -189,6 +189,27 @@ hegj_status gather_rows(const std::vector<uint32_t *> &batches, const std::vecto
     return HEGJ_OK;
 }
 
+namespace {
+__global__ __launch_bounds__(kT) void k_scatter_rows(const uint32_t *__restrict__ qid,
+                                                     const uint64_t *__restrict__ off,
+                                                     const uint32_t *__restrict__ cnt, uint32_t m,
+                                                     uint64_t base, uint64_t *__restrict__ row_off,
+                                                     uint32_t *__restrict__ row_cnt) {
+    const uint32_t i = blockIdx.x * kT + threadIdx.x;
+    if (i >= m) return;
+    row_off[qid[i]] = base + off[i];
+    row_cnt[qid[i]] = cnt[i];
+}
+}  // namespace
+
+hegj_status scatter_rows(const uint32_t *qid, const uint64_t *off, const uint32_t *cnt, uint32_t m,
+                         uint64_t base, uint64_t *row_off, uint32_t *row_cnt, hipStream_t s) {
+    if (m == 0) return HEGJ_OK;
+    k_scatter_rows<<<ceil_div(m, kT), kT, 0, s>>>(qid, off, cnt, m, base, row_off, row_cnt);
+    HIP_TRY(hipGetLastError());
+    return HEGJ_OK;
+}
+
 }  // namespace hegj
 
 namespace hegj {
@@ -402,7 +423,8 @@ hegj_status make_soa32(const GridGeom &g, const DevIndex &ix, float **soa32, flo
             lo = std::min(lo, hp[((size_t)b * dim + l) * 2]);
             hi = std::max(hi, hp[((size_t)b * dim + l) * 2 + 1]);
         }
-        c[l] = lo + (hi - lo) * 0.5;
+        // DTYPE=float: no centring, the copy is the float data itself
+        c[l] = g.f32dt ? 0.0 : lo + (hi - lo) * 0.5;
         R = std::max(R, std::max(hi - c[l], c[l] - lo));
     }
     *radius = R * (1.0 + 1e-12);

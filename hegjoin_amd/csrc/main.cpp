@@ -1,11 +1,13 @@
 // main.cpp -- `./main dataset epsilon dimensionality algorithm [static_partitioning]`
 //
-// Drop-in for the reference CLI (main.cu:42-446, README.md:57-74) for
-// algorithm 0 (GPU only).  Argument checks and exit codes follow
-// main.cu:47-71; the [RESULT] lines follow main.cu:366-407, GPU.cu:1622-1628
-// and SortByWorkload.cu:220-221 so `| grep RESULT` pipelines keep working.
-// Algorithms 1-3 (hybrid CPU+GPU, CPU only) are not part of this build and
-// are rejected with exit code 1.
+// Drop-in for the reference CLI (main.cu:42-446, README.md:57-74): algorithm
+// 0 (GPU only, hegj_selfjoin), 1 (hybrid: the GPU takes the work queue's head,
+// Super-EGO CPU threads its tail), 2 (static split: the fifth argument, or the
+// model of StaticPartition.cpp) and 3 (Super-EGO alone), through
+// hegj_selfjoin_hybrid.  Argument checks and exit codes follow main.cu:47-71
+// and 131-140; the [RESULT] lines follow main.cu:366-407, GPU.cu:635,
+// 1622-1628, MultiThreadJoin.cpp:234-236 and SortByWorkload.cu:220-221 so
+// `| grep RESULT` pipelines keep working.
 //
 // Compile-time parameters keep the params.h semantics: -DGPUNUMDIM=<d>
 // makes the CLI insist that dim == GPUNUMDIM (main.cu:61-65);
@@ -19,6 +21,10 @@
 //   --tuning=PATH       join options per dimensionality (autotuner table);
 //                       by default tuning.tsv next to this executable is used
 //                       when present
+//   --cpu-threads=T     Super-EGO threads (CPU_THREADS) of algorithms 1-3
+//   --bin-cache         keep a binary copy of a parsed text dataset beside it
+//                       (<dataset>.d<dim>.hegjbin) and read that copy while
+//                       the text file is unchanged
 #include <cerrno>
 #include <chrono>
 #include <cstdint>
@@ -33,6 +39,16 @@
 
 #ifndef NUMINDEXEDDIM
 #define NUMINDEXEDDIM 6
+#endif
+// DTYPE (params.h:24): -DDTYPE=float builds the CLI for float data, as the
+// reference's PARAMS hook does; --dtype=float|double overrides at run time.
+#ifndef DTYPE
+#define DTYPE double
+#endif
+// CPU_THREADS (params.h:14): Super-EGO threads of algorithms 1-3; 0 = every
+// CPU.  --cpu-threads=T overrides at run time.
+#ifndef CPU_THREADS
+#define CPU_THREADS 0
 #endif
 
 #define NB_ARGS_MAX 6
@@ -49,6 +65,9 @@ int main(int argc, char **argv) {
     uint32_t kOverride = 0;
     int emulate = 1, device = 0;
     const char *dump = nullptr, *tuning = nullptr;
+    uint32_t load_flags = 0;
+    bool use_float = sizeof(DTYPE) == sizeof(float);
+    uint32_t cpu_threads = CPU_THREADS;
     for (int i = 1; i < argc; i++) {
         if (std::strncmp(argv[i], "--indexed-dims=", 15) == 0)
             kOverride = (uint32_t)std::atoi(argv[i] + 15);
@@ -60,6 +79,14 @@ int main(int argc, char **argv) {
             device = std::atoi(argv[i] + 9);
         else if (std::strncmp(argv[i], "--tuning=", 9) == 0)
             tuning = argv[i] + 9;
+        else if (std::strcmp(argv[i], "--bin-cache") == 0)
+            load_flags |= HEGJ_LOAD_BIN_CACHE;
+        else if (std::strncmp(argv[i], "--cpu-threads=", 14) == 0)
+            cpu_threads = (uint32_t)std::atoi(argv[i] + 14);
+        else if (std::strcmp(argv[i], "--dtype=float") == 0)
+            use_float = true;
+        else if (std::strcmp(argv[i], "--dtype=double") == 0)
+            use_float = false;
         else
             pos.push_back(argv[i]);
     }
@@ -109,17 +136,30 @@ int main(int argc, char **argv) {
     std::printf("[MAIN] ~ Epsilon: %f\n", epsilon);
     std::printf("[MAIN] ~ Dimensionality: %d\n", dim);
     std::printf("[MAIN] ~ Search mode: %d\n", searchMode);
-    if (searchMode != 0) {
-        std::fprintf(stderr,
-                     "[MAIN] ~ Error: this build provides algorithm 0 (GPU only); "
-                     "algorithms 1-3 (hybrid / CPU Super-EGO) are not built\n");
+    if (searchMode < 0 || searchMode > 3) {
+        std::fprintf(stderr, "[MAIN] ~ Error: algorithm must be 0 (GPU), 1 (hybrid), 2 (static hybrid) or 3 (CPU)\n");
         return 1;
     }
+    // the static partition (main.cu:131-168): from the command line, or the model
+    float staticPartition = -1.0f;
+    if (searchMode == 2 && nargs == NB_ARGS_MAX) {
+        staticPartition = (float)std::atof(pos[4]);
+        if (staticPartition <= 0.0f || 1.0f <= staticPartition) {
+            std::fprintf(stderr, "[MAIN] ~ Error: the partitioning should be between ]0.0, 1.0[");
+            return 1;
+        }
+        std::printf("[MODEL | RESULT] ~ Statically partitioning the work, using the partitioning value %f\n",
+                    staticPartition);
+    }
 
+    if (use_float) {
+        load_flags |= HEGJ_LOAD_F32;
+        epsilon = (double)(float)epsilon;  // DTYPE epsilon = atof(...) (main.cu:56)
+    }
     double t0 = now_s();
     double *data = nullptr;
     uint32_t n = 0;
-    if (hegj_dataset_load(filename, (uint32_t)dim, &data, &n) != HEGJ_OK) {
+    if (hegj_dataset_load_ex(filename, (uint32_t)dim, load_flags, &data, &n) != HEGJ_OK) {
         std::fprintf(stderr, "[MAIN] ~ Error: %s\n", hegj_last_error());
         return 1;
     }
@@ -136,6 +176,7 @@ int main(int argc, char **argv) {
     o.indexed_dims = k < (uint32_t)dim ? k : (uint32_t)dim;
     o.emulate_cell0 = (uint32_t)emulate;
     o.device = device;
+    o.dtype = use_float ? HEGJ_DTYPE_F32 : HEGJ_DTYPE_F64;
 
     double tIndex = now_s();
     hegj_index *idx = nullptr;
@@ -170,29 +211,73 @@ int main(int argc, char **argv) {
                     (unsigned long long)(totalCandidates / n));
     std::printf("\n\n[MAIN] ~ Time to do everything before computing: %f\n\n\n", now_s() - tStartStart);
 
-    double tBeginGPU = now_s();
+    if (searchMode == 2 && staticPartition < 0.0f) {
+        staticPartition = (float)hegj_static_partition_model(n, (uint32_t)dim, epsilon);
+        std::printf("[MODEL | RESULT] ~ Modeled GPU partition: %f, CPU partition: %f\n", staticPartition,
+                    1 - staticPartition);
+    }
+    double tStart = now_s();
     hegj_result *res = nullptr;
-    st = hegj_selfjoin(idx, nullptr, &res);
+    if (searchMode == 0) {
+        st = hegj_selfjoin(idx, nullptr, &res);
+    } else {
+        hegj_hybrid_opts ho;
+        hegj_hybrid_opts_default(&ho, (uint32_t)searchMode);
+        ho.cpu_threads = cpu_threads;
+        ho.gpu_fraction = staticPartition;
+        st = hegj_selfjoin_hybrid(idx, nullptr, &ho, &res);
+    }
     if (st != HEGJ_OK) {
         std::fprintf(stderr, "[GPU] ~ Error: %s: %s\n", hegj_status_string(st), hegj_last_error());
         return 1;
     }
-    double gpuTime = now_s() - tBeginGPU;
+    const double computeTime = now_s() - tStart;
     hegj_result_info ri;
     hegj_result_get_info(res, &ri);
-    std::printf("[GPU] ~ Kernel execution time: %f ms over %u launch(es), %u batch(es)\n",
-                ri.join_ms, ri.join_launches, ri.n_batches);
-    std::printf("[GPU | RESULT] ~ Query points computed by the GPU: %u (f: %f)\n", n, 1.0);
-    std::printf("[GPU | RESULT] ~ Compute time for the GPU: %f\n", gpuTime);
+    // the GPU / CPU split: everything on the GPU in algorithm 0, where the CPU
+    // takes no work and so ends when the join starts
+    hegj_hybrid_info hi{};
+    if (searchMode == 0) {
+        hi.n_queries_gpu = n;
+        hi.pairs_gpu = ri.total_neighbors;
+        hi.gpu_s = hi.gpu_end_s = computeTime;
+    } else {
+        hegj_result_get_hybrid_info(res, &hi);
+    }
+    const double gpuTime = searchMode == 3 ? 0.0 : hi.gpu_s;
+    if (searchMode != 3) {
+        std::printf("[GPU | RESULT] ~ Total estimated workload: %llu\n", (unsigned long long)ri.estimated_pairs);
+        std::printf("[GPU] ~ Kernel execution time: %f ms over %u launch(es), %u batch(es)\n", ri.join_ms,
+                    ri.join_launches, ri.n_batches);
+        std::printf("[GPU | RESULT] ~ Query points computed by the GPU: %u (f: %f)\n", hi.n_queries_gpu,
+                    (double)hi.n_queries_gpu / (double)n);
+        std::printf("[GPU | RESULT] ~ Compute time for the GPU: %f\n", gpuTime);
+    }
+    if (searchMode != 0) {
+        // MultiThreadJoin.cpp:234-236
+        std::printf("[EGO | RESULT] ~ Query points computed by Super-EGO: %u\n", hi.n_queries_cpu);
+        std::printf("[EGO | RESULT] ~ Candidate points refined by Super-EGPU: %llu\n",
+                    (unsigned long long)hi.candidates_cpu);
+        std::printf("[EGO | RESULT] ~ Compute time for Super-EGO: %f\n", hi.cpu_s);
+    }
 
     std::printf("[RESULT] ~ Total result set size: %lu\n", (unsigned long)ri.total_neighbors);
-    std::printf("   [RESULT] ~ Total result set size on the GPU: %lu\n", (unsigned long)ri.total_neighbors);
-    std::printf("   [RESULT] ~ Total result set size on the CPU: %lu\n", 0ul);
-    std::printf("[RESULT] ~ Total execution time: %f\n", gpuTime + sortTime);
+    std::printf("   [RESULT] ~ Total result set size on the GPU: %lu\n", (unsigned long)hi.pairs_gpu);
+    std::printf("   [RESULT] ~ Total result set size on the CPU: %lu\n", (unsigned long)hi.pairs_cpu);
+    std::printf("[RESULT] ~ Total execution time: %f\n", computeTime + sortTime);
     std::printf("   [RESULT] ~ Total execution time to SortByWL: %f\n", sortTime);
     std::printf("   [RESULT] ~ Total execution time for the GPU: %f\n", gpuTime);
     std::printf("   [RESULT] ~ Total execution time for the CPU: %f (reorder: %f, sort: %f, total = %f)\n",
-                0.0, 0.0, 0.0, 0.0);
+                hi.cpu_s, hi.reorder_s, hi.sort_s, hi.cpu_s + hi.reorder_s + hi.sort_s);
+    // main.cu:400-407
+    const double total = computeTime + sortTime;
+    const double gpu_end = searchMode == 3 ? 0.0 : hi.gpu_end_s, cpu_end = hi.cpu_end_s;
+    if (gpu_end < cpu_end)
+        std::printf("[RESULT] ~ The GPU ended before the CPU, with a difference of: %f (ratio: %f)\n",
+                    cpu_end - gpu_end, total > 0 ? (cpu_end - gpu_end) / total : 0.0);
+    else
+        std::printf("[RESULT] ~ The CPU ended before the GPU, with a difference of: %f (ratio: %f)\n",
+                    gpu_end - cpu_end, total > 0 ? (gpu_end - cpu_end) / total : 0.0);
     std::printf("[RESULT] ~ Pair multiset hash: %016llx\n", (unsigned long long)ri.pair_hash);
 
     int rc = 0;

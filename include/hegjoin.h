@@ -99,6 +99,20 @@ hegj_status hegj_dataset_generate_expo(uint32_t seed, uint32_t n, uint32_t dim, 
  * numbers, n = values / dim; *.bin = raw row-major doubles.  *out is freed
  * with hegj_dataset_free. */
 hegj_status hegj_dataset_load(const char *path, uint32_t dim, double **out, uint32_t *n);
+/* hegj_dataset_load with options (0 = hegj_dataset_load):
+ *   HEGJ_LOAD_F32       numbers are parsed as floats (`ss >> float` with
+ *                       DTYPE=float, import_dataset.cpp:17-19), returned as
+ *                       the doubles equal to those floats;
+ *   HEGJ_LOAD_BIN_CACHE a text file is parsed once: its values go to the
+ *                       binary cache <path>.d<dim>[f].hegjbin beside it
+ *                       (header with the text file's size and mtime, then
+ *                       row-major doubles), which later loads read while the
+ *                       text file is unchanged.  Binary datasets
+ *                       (import_dataset.cpp:63-112) are the *.bin files. */
+#define HEGJ_LOAD_BIN_CACHE 1u
+#define HEGJ_LOAD_F32 2u
+hegj_status hegj_dataset_load_ex(const char *path, uint32_t dim, uint32_t flags, double **out,
+                                 uint32_t *n);
 void hegj_dataset_free(double *p);
 
 /* ------------------------------------------------------------------------ */
@@ -109,7 +123,14 @@ typedef struct hegj_index hegj_index;
 typedef struct {
     uint32_t dim;           /* GPUNUMDIM (params.h:5) */
     uint32_t indexed_dims;  /* NUMINDEXEDDIM (params.h:8), 1..HEGJ_MAX_INDEXED_DIMS */
-    uint32_t dtype;         /* hegj_dtype; DTYPE (params.h:24) */
+    uint32_t dtype;         /* hegj_dtype; DTYPE (params.h:24).  HEGJ_DTYPE_F32: the
+                               database and epsilon are rounded to float and
+                               minArr, nCells, cell ids (main.cu:493-510,
+                               kernel.cu:54-57) and the distance chain with its
+                               sqrtf predicate (kernel.cu:347-355) are computed in
+                               FP32, as the reference built with DTYPE=float.
+                               The MFMA refine and graph replay are FP64 only;
+                               refine FP64 then names the plain (unfiltered) kernel. */
     uint32_t emulate_cell0; /* 1 = reproduce the reference's cell-0 wrap (kernel.cu:780) */
     int32_t device;         /* HIP device ordinal */
 } hegj_index_opts;
@@ -237,8 +258,97 @@ typedef struct {
     uint32_t refine_used;      /* hegj_refine that ran: FP64 or MFMA */
     uint32_t graph_replay;     /* 1: the join ran as a replay of the captured graph */
     uint32_t reserved;
+    uint64_t estimated_pairs;  /* the batch estimator's pair estimate ("[GPU | RESULT] ~
+                                  Total estimated workload", GPU.cu:635); the MFMA
+                                  refine sizes its output exactly, so it reports that */
 } hegj_result_info;
 hegj_status hegj_result_get_info(const hegj_result *res, hegj_result_info *info);
+
+/* ------------------------------------------------------------------------
+ * Hybrid CPU + GPU self-join: algorithms 1, 2 and 3 of
+ * `./main dataset epsilon dim algorithm [static_partitioning]`
+ * (distanceTableNDGridBatches' SM_HYBRID branch GPU.cu:1179-1363, WorkQueue.cpp
+ * :31-118, MultiThreadJoin.cpp:21-115, main.cu:131-187, 255-362).
+ *
+ * The queries, in the workload order of hegj_sort_by_workload
+ * (originPointIndex), form one queue of work units (up to queries_per_unit
+ * queries of one cell).  The GPU takes chunks of units from the head of the
+ * queue and runs each through the persistent join kernel into the result's
+ * device batches; CPU threads take batches of about `cpu_batch` queries from
+ * the tail (getBatchFromQueueCPU, WorkQueue.cpp:97-118) and answer each query
+ * with Super-EGO over the whole dataset (Util::egoJoinV2).  The queue's head
+ * and tail are host-side indices under one lock; the GPU never waits for the
+ * CPU nor the reverse.  A query answered on the GPU gets its algorithm-0 row,
+ * one answered on the CPU its Super-EGO row (no cell-0 wrap, Super-EGO's own
+ * rounding), as in the reference.  The CPU rows are copied into the result,
+ * so every hegj_result_* call sees one table.
+ *   HEGJ_MODE_HYBRID  the split is dynamic (whoever reaches a unit first);
+ *   HEGJ_MODE_STATIC  the GPU takes the first gpu_fraction of the queries (at
+ *                     a unit boundary), the CPU the rest; gpu_fraction <= 0
+ *                     uses hegj_static_partition_model;
+ *   HEGJ_MODE_CPU     every query on the CPU (algorithm 3).
+ * ------------------------------------------------------------------------ */
+typedef enum {
+    HEGJ_MODE_GPU = 0,     /* SM_GPU (params.h:58): hegj_selfjoin */
+    HEGJ_MODE_HYBRID = 1,  /* SM_HYBRID */
+    HEGJ_MODE_STATIC = 2,  /* SM_HYBRID_STATIC */
+    HEGJ_MODE_CPU = 3      /* SM_CPU */
+} hegj_search_mode;
+
+typedef struct {
+    uint32_t mode;          /* hegj_search_mode */
+    uint32_t cpu_threads;   /* Super-EGO threads (CPU_THREADS, params.h:14); 0 = every CPU */
+    uint32_t cpu_batch;     /* queries per CPU batch (CPU_BATCH_SIZE, params.h:17); 0 = 1024 */
+    uint32_t gpu_chunk;     /* work units per GPU launch in HEGJ_MODE_HYBRID; 0 = 1/16 of them */
+    double gpu_fraction;    /* HEGJ_MODE_STATIC: the GPU's share of the queries, in ]0, 1[ */
+} hegj_hybrid_opts;
+
+typedef struct {
+    uint32_t mode;
+    uint32_t cpu_threads;
+    uint32_t n_queries_gpu;   /* "[GPU | RESULT] ~ Query points computed by the GPU" */
+    uint32_t n_queries_cpu;   /* "[EGO | RESULT] ~ Query points computed by Super-EGO" */
+    uint64_t pairs_gpu;       /* "[RESULT] ~ Total result set size on the GPU" */
+    uint64_t pairs_cpu;       /* "[RESULT] ~ Total result set size on the CPU" */
+    uint64_t candidates_cpu;  /* points Super-EGO scanned */
+    double gpu_fraction;      /* HEGJ_MODE_STATIC: the split used */
+    double gpu_s;             /* GPU side: first launch -> last batch done */
+    double cpu_s;             /* CPU side: Super-EGO queries */
+    double reorder_s, sort_s; /* CPU side: dimension reorder, EGO sort */
+    double gpu_end_s, cpu_end_s; /* each side's end, from the start of the call */
+} hegj_hybrid_info;
+
+/* Defaults for a mode: every CPU thread, batches of 1024, automatic chunks,
+ * gpu_fraction 0 (the model). */
+void hegj_hybrid_opts_default(hegj_hybrid_opts *opts, uint32_t mode);
+/* opts as for hegj_selfjoin (NULL: tuned defaults; the MFMA refine is not
+ * used: the GPU runs the grid join kernels).  HEGJ_MODE_GPU is
+ * hegj_selfjoin. */
+hegj_status hegj_selfjoin_hybrid(hegj_index *idx, const hegj_join_opts *opts,
+                                 const hegj_hybrid_opts *hopts, hegj_result **out);
+/* The CPU/GPU figures of a hybrid result (HEGJ_ERR_INVALID_ARG for others). */
+hegj_status hegj_result_get_hybrid_info(const hegj_result *res, hegj_hybrid_info *info);
+
+/* Super-EGO on the host alone: the CPU share of the hybrid modes
+ * (Util::reorderDim, the EGO sort of main.cu:296-318 and Util::egoJoinV2 per
+ * query, MultiThreadJoin.cpp:43-115), for `threads` threads (0 = every CPU).
+ * Rows of the queries qids[0..nq) (qids NULL: every point, nq = n) over all n
+ * points: counts[nq], *nbrs = the rows back to back (each in EGO order;
+ * freed with hegj_free), *total = pairs.  Needs no device. */
+hegj_status hegj_superego_rows(const double *database, uint32_t n, uint32_t dim, double epsilon,
+                               uint32_t threads, const uint32_t *qids, uint32_t nq,
+                               uint32_t *counts, uint32_t **nbrs, uint64_t *total);
+void hegj_free(void *p);
+
+/* Static split model (StaticPartition.cpp:67-108, main.cu:141-168): modelled
+ * times t = exp(c0 + c1 ln(dim) + c2 ln(n) + c3 ln(eps)) seconds for the GPU
+ * and the CPU; the GPU share is (n / t_gpu) / (n / t_gpu + n / t_cpu), each
+ * rate truncated to an integer as main.cu:152-153 does.  The coefficients
+ * are the reference's (fitted on its A100 + Zen 2 host) until replaced with
+ * hegj_static_partition_set_model. */
+double hegj_static_partition_model(uint32_t n, uint32_t dim, double epsilon);
+/* gpu[4], cpu[4] = (c0, c1, c2, c3) of the two time models. */
+void hegj_static_partition_set_model(const double *gpu, const double *cpu);
 
 /* Per-query neighbour counts, original point ids (neighborTable row sizes). */
 hegj_status hegj_result_copy_counts(const hegj_result *res, uint32_t *counts);

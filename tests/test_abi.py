@@ -53,6 +53,9 @@ PROBE = r"""
 int main(void) {
     S(hegj_index_opts) S(hegj_index_info) S(hegj_join_opts) S(hegj_result_info)
     S(hegj_serve_opts) S(hegj_request_info) S(hegj_serve_stats)
+    S(hegj_hybrid_opts) S(hegj_hybrid_info)
+    O(hegj_hybrid_opts, gpu_fraction) O(hegj_hybrid_info, candidates_cpu) O(hegj_hybrid_info, cpu_end_s)
+    O(hegj_result_info, estimated_pairs)
     O(hegj_index_info, build_ms) O(hegj_join_opts, part_by_block) O(hegj_join_opts, use_graph)
     O(hegj_join_opts, no_prune)
     O(hegj_result_info, refine_used) O(hegj_result_info, graph_replay) O(hegj_serve_opts, result_bytes)
@@ -76,7 +79,8 @@ def test_ctypes_mirrors_match_c_layouts(tmp_path):
     mirror = {"hegj_index_opts": H._IndexOpts, "hegj_index_info": H._IndexInfo,
               "hegj_join_opts": H._JoinOpts, "hegj_result_info": H._ResultInfo,
               "hegj_serve_opts": H._ServeOpts, "hegj_request_info": H._RequestInfo,
-              "hegj_serve_stats": H._ServeStats}
+              "hegj_serve_stats": H._ServeStats, "hegj_hybrid_opts": H._HybridOpts,
+              "hegj_hybrid_info": H._HybridInfo}
     for name, cls in mirror.items():
         assert int(got[name]) == C.sizeof(cls), name
     for key, val in got.items():
@@ -152,8 +156,8 @@ def test_argument_validation_statuses():
         st = lib.hegj_index_build(pts.ctypes.data, 10, eps, C.byref(o), C.byref(h))
         assert st == 1, (eps, dim, k)
     lib.hegj_index_opts_default(C.byref(o), 2)
-    o.dtype = 1
-    assert lib.hegj_index_build(pts.ctypes.data, 10, 0.1, C.byref(o), C.byref(h)) == 2
+    o.dtype = 2  # neither HEGJ_DTYPE_F64 nor HEGJ_DTYPE_F32
+    assert lib.hegj_index_build(pts.ctypes.data, 10, 0.1, C.byref(o), C.byref(h)) == 1
     assert lib.hegj_index_build(None, 10, 0.1, C.byref(o), C.byref(h)) == 1
     assert lib.hegj_selfjoin(None, None, C.byref(h)) == 1
     assert lib.hegj_index_free(None) == 0
@@ -164,8 +168,9 @@ def test_argument_validation_statuses():
     (["x.txt", "0.1", "2"], "Expected 6 or 5 args"),
     (["x.txt", "1.5", "2", "0"], "epsilon should be between"),
     (["x.txt", "0", "2", "0"], "epsilon should be between"),
-    (["x.txt", "0.1", "2", "3"], "algorithm 0"),
-    (["x.txt", "0.1", "2", "1", "0.5"], "algorithm 0"),
+    (["x.txt", "0.1", "2", "4"], "algorithm must be"),
+    (["x.txt", "0.1", "2", "2", "1.5"], "partitioning should be between"),
+    (["x.txt", "0.1", "2", "2", "0"], "partitioning should be between"),
 ])
 def test_cli_argument_contract(args, msg):
     r = subprocess.run([CLI, *args], capture_output=True, text=True)
@@ -303,3 +308,77 @@ def test_release_cached_memory_is_safe_without_a_device():
     import hegjoin_amd as H
     H.release_cached_memory()
     H.release_cached_memory(0)
+
+
+def test_dataset_binary_cache(oracle, tmp_path):
+    """HEGJ_LOAD_BIN_CACHE: the first load parses the text and writes
+    <path>.d<dim>.hegjbin; later loads read that file while the text is
+    unchanged (shown by planting a value in the cache), and re-parse once the
+    text changes."""
+    import hegjoin_amd as H
+    pts = oracle.generate_expo(5000, 3, seed=5)
+    p = str(tmp_path / "d.txt")
+    oracle.write_text(p, pts, 17)
+    a = H.load_dataset(p, 3, bin_cache=True)
+    assert np.array_equal(a, pts)
+    cache = p + ".d3.hegjbin"
+    raw = bytearray(open(cache, "rb").read())
+    assert raw[:8] == b"HEGJBIN1" and len(raw) == 56 + pts.nbytes
+    planted = np.frombuffer(bytes(raw[56:]), np.float64).copy()
+    planted[7] = 123.0
+    raw[56:] = planted.tobytes()
+    open(cache, "wb").write(bytes(raw))
+    b = H.load_dataset(p, 3, bin_cache=True)
+    assert b.reshape(-1)[7] == 123.0  # read from the cache, not re-parsed
+    assert np.array_equal(H.load_dataset(p, 3), pts)  # no flag: the text
+    with open(p, "a") as f:  # the text changes: the cache is stale
+        f.write("0.5,0.25,0.125\n")
+    c = H.load_dataset(p, 3, bin_cache=True)
+    assert c.shape == (5001, 3) and np.array_equal(c[:5000], pts)
+    assert np.array_equal(H.load_dataset(p, 3, bin_cache=True), c)
+    # float parsing keeps its own cache
+    f = H.load_dataset(p, 3, f32=True, bin_cache=True)
+    assert os.path.exists(p + ".d3f.hegjbin")
+    assert f.shape == c.shape
+
+
+def _decimal(fr, digits=60):
+    """Exact decimal text of a dyadic fraction in [0, 1) (60 digits suffice
+    for the values below)."""
+    q = fr * 10**digits
+    assert q.denominator == 1
+    t = str(q.numerator).rjust(digits + 1, "0")
+    return t[:-digits] + "." + t[-digits:]
+
+
+def test_dataset_float_parse_is_correctly_rounded(tmp_path):
+    """HEGJ_LOAD_F32 (`ss >> float` with DTYPE=float): every value is the
+    float nearest to the decimal text, ties to even, checked with exact
+    rational arithmetic.  The cases include exact midpoints between adjacent
+    floats and midpoints nudged by 1e-45, which a parse through an
+    intermediate double rounds the wrong way."""
+    from fractions import Fraction
+    import hegjoin_amd as H
+    rng = np.random.default_rng(9)
+    vals = [repr(float(v)) for v in rng.random(2000)]
+    ties = set()
+    for v in rng.random(300).astype(np.float32):
+        lo = Fraction(float(v))
+        mid = (lo + Fraction(float(np.nextafter(v, np.float32(2))))) / 2
+        ties.add(len(vals))
+        vals.append(_decimal(mid))
+        vals.append(_decimal(mid + Fraction(1, 10**45)))
+        vals.append(_decimal(mid - Fraction(1, 10**45)))
+    p = str(tmp_path / "f.txt")
+    open(p, "w").write("\n".join(vals) + "\n")
+    got = H.load_dataset(p, 1, f32=True).reshape(-1)
+    assert len(got) == len(vals)
+    for i, (s, g) in enumerate(zip(vals, got)):
+        x = Fraction(s)
+        g32 = np.float32(g)
+        assert float(g32) == g
+        err = abs(Fraction(float(g32)) - x)
+        for nb in (np.nextafter(g32, np.float32(-2)), np.nextafter(g32, np.float32(2))):
+            assert err <= abs(Fraction(float(nb)) - x), s
+        if i in ties:  # exactly halfway: the even neighbour
+            assert int(g32.view(np.uint32)) % 2 == 0, s

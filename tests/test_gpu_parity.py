@@ -484,3 +484,78 @@ def test_cached_buffers_released_between_joins(H, oracle):
             H.release_cached_memory()
         off, nb, _ = gpu_join(H, pts, 0.015, 6)
         assert np.array_equal(off, eoff) and np.array_equal(nb, enb)
+
+
+def planted_pairs(pts, eps, seed, m=400):
+    """m partners of the first m points at distance eps (1 + r), r in
+    {+-3e-8, +-1e-8, 0}: near the threshold, where FP32 and FP64 decide
+    differently, so a float join that ran in FP64 (or the reverse) fails."""
+    rng = np.random.default_rng(seed)
+    d = rng.standard_normal((m, pts.shape[1]))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rel = rng.choice([-3e-8, -1e-8, 0.0, 1e-8, 3e-8], m)
+    part = pts[:m] + d * (eps * (1 + rel))[:, None]
+    return np.ascontiguousarray(np.concatenate([pts, part]))
+
+
+# (dim, k, n, eps, refine, queries per unit): the FP32-filter instances with
+# the float data itself (dim 2 and 6) and the generic FP32-chain kernel
+# (refine fp64 = the plain kernel, dim 3 / 18, 6-D with 16 queries per unit)
+FLOAT_CASES = [(2, 2, 20000, 0.01, None, None), (2, 2, 20000, 0.01, "fp64", None),
+               (2, 2, 20000, 0.01, "f32", 16), (6, 6, 20000, 0.06, None, None),
+               (6, 6, 20000, 0.06, "fp64", 4), (6, 6, 20000, 0.06, "auto", 16),
+               (3, 3, 20000, 0.03, None, None), (18, 6, 4000, 0.25, None, None)]
+
+
+@pytest.mark.parametrize("dim,k,n,eps,refine,qb", FLOAT_CASES,
+                         ids=[f"d{c[0]}-{c[4] or 'auto'}-qb{c[5] or 0}" for c in FLOAT_CASES])
+def test_dtype_float_vs_oracle(H, oracle, dim, k, n, eps, refine, qb):
+    """DTYPE=float (params.h:24): database and epsilon rounded to float,
+    minArr / nCells / cell ids and the distance chain in FP32 -- the oracle's
+    Grid<float> restatement.  The grid is also checked against numpy float32
+    arithmetic (IEEE, correctly rounded) and the planted near-eps pairs make
+    the FP32 and FP64 pair sets differ, so the test tells the two apart."""
+    print("case float", dim, k, n, eps, refine, qb, flush=True)
+    pts = planted_pairs(H.generate_expo(n, dim, seed=11 + dim), eps, seed=dim)
+    e64 = oracle.join_stats(pts, k, eps)
+    e32 = oracle.join_stats(pts, k, eps, f32=True)
+    assert e32 != e64  # the planted pairs separate the two precisions
+    kw = {}
+    if refine:
+        kw["refine"] = refine
+    if qb:
+        kw["queries_per_unit"] = qb
+    with H.GridIndex(pts, eps, indexed_dims=k, dtype="f32") as ix:
+        info = ix.info
+        ids = ix.cell_ids()
+        t = ix.selfjoin(**kw)
+        off, nb = t.csr(sort_rows=True)
+        ri = t.info
+        t.close()
+    # grid in float32 arithmetic (main.cu:463-510, kernel.cu:54-57)
+    p32 = pts.astype(np.float32)
+    e = np.float32(eps)
+    mn = p32[:, :k].min(0) - e
+    mx = p32[:, :k].max(0) + e
+    nc = np.ceil((mx - mn) / e).astype(np.uint32)
+    assert info["dtype"] == 1 and info["epsilon"] == float(e)
+    assert info["minArr"] == mn.astype(np.float64).tolist()
+    assert info["nCells"] == nc.tolist()
+    c = ((p32[:, :k] - mn) / e).astype(np.uint64)
+    mult = np.concatenate([[1], np.cumprod(nc.astype(np.uint64))[:-1]]).astype(np.uint64)
+    assert np.array_equal(ids, (c * mult).sum(1).astype(np.uint64))
+    assert info["threshold"] == float(oracle.threshold_f(eps))
+    eoff, enb = oracle.join(pts, k, eps, f32=True)
+    assert np.array_equal(off, eoff) and np.array_equal(nb, enb)
+    assert (ri["total_neighbors"], ri["pair_hash"]) == e32
+    assert ri["refine_used"] == ("f32" if (dim in (2, 6) and refine != "fp64"
+                                           and not (dim == 6 and qb == 16)) else "fp64")
+    assert ri["band_rechecks"] == 0
+
+
+def test_dtype_float_refuses_mfma(H):
+    pts = H.generate_expo(2000, 18, seed=1)
+    with H.GridIndex(pts, 0.3, dtype="f32") as ix:
+        with pytest.raises(H.HegjError) as e:
+            ix.selfjoin(refine="mfma")
+    assert e.value.status == 2
