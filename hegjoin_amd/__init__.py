@@ -163,6 +163,8 @@ _lib.hegj_f32_thresholds.argtypes = [C.c_double, C.c_uint32, C.c_double, C.POINT
 _lib.hegj_join_opts_tuned.argtypes = [C.POINTER(_JoinOpts), C.c_uint32]
 _lib.hegj_tuning_set.argtypes = [C.c_uint32, C.POINTER(_JoinOpts)]
 _lib.hegj_tuning_load.argtypes = [C.c_char_p, C.POINTER(C.c_uint32)]
+_lib.hegj_selfjoin_noalloc.argtypes = [_vp, C.POINTER(_JoinOpts), _vp, C.c_uint64, _vp, _vp,
+                                       C.POINTER(_ResultInfo)]
 _lib.hegj_hybrid_opts_default.argtypes = [C.POINTER(_HybridOpts), C.c_uint32]
 _lib.hegj_hybrid_opts_default.restype = None
 _lib.hegj_selfjoin_hybrid.argtypes = [_vp, C.POINTER(_JoinOpts), C.POINTER(_HybridOpts),
@@ -425,6 +427,21 @@ class GridIndex:
         r = _vp()
         _check(_lib.hegj_selfjoin_hybrid(self._h, C.byref(o), C.byref(h), C.byref(r)))
         return NeighborTable(r, self)
+
+    def selfjoin_noalloc(self, neighbors, capacity, row_off, row_cnt, **join_kw):
+        """hegj_selfjoin_noalloc: the self-join into caller-owned device
+        buffers (integer device addresses) neighbors[capacity] (uint32),
+        row_off[n] (uint64), row_cnt[n] (uint32), with no device allocation.
+        Returns the result info; HegjError(6) when the rows do not fit."""
+        o = self._join_opts(grid_only=True, **join_kw)
+        ri = _ResultInfo()
+        st = _lib.hegj_selfjoin_noalloc(self._h, C.byref(o), neighbors, capacity, row_off, row_cnt,
+                                        C.byref(ri))
+        if st != 0:
+            err = HegjError(st, _lib.hegj_last_error().decode())
+            err.info = _struct_dict(ri)
+            raise err
+        return _struct_dict(ri)
 
     def _join_opts(self, result_capacity=0, unit_order=None, queries_per_unit=None,
                    xcd_parts=None, refine=None, units_per_grab=None, part_by_block=0,

@@ -25,7 +25,7 @@ ARCH = os.environ.get("HEGJ_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["hegj_prims.hip", "hegj_index.hip", "hegj_join.hip", "hegj_util.hip",
            "hegj_mfma.hip", "hegj_query.hip", "hegj_serve.cpp", "hegj_dataset.cpp", "hegj_ego.cpp",
-           "hegj_api.cpp"]
+           "hegj_scratch.hip", "hegj_api.cpp"]
 
 # -packed-fp32-ops: keep the compiler from emitting packed FP32 instructions.
 HIPFLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-Wall",

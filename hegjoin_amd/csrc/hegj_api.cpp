@@ -34,14 +34,20 @@ static thread_local Arena *g_arena = nullptr;
 
 void set_thread_arena(Arena *a) { g_arena = a; }
 
+static hipError_t arena_alloc(void **p, size_t bytes) {
+    const size_t off = (g_arena->used + 255) & ~(size_t)255;
+    if (off > g_arena->cap || bytes > g_arena->cap - off) return hipErrorOutOfMemory;
+    *p = g_arena->base + off;
+    g_arena->used = off + bytes;
+    return hipSuccess;
+}
+
+static bool in_arena(const void *p) {
+    return g_arena && (const char *)p >= g_arena->base && (const char *)p < g_arena->base + g_arena->cap;
+}
+
 hipError_t Scratch::alloc(void **p, size_t bytes) {
-    if (g_arena) {
-        const size_t off = (g_arena->used + 255) & ~(size_t)255;
-        if (off + bytes > g_arena->cap) return hipErrorOutOfMemory;
-        *p = g_arena->base + off;
-        g_arena->used = off + bytes;
-        return hipSuccess;
-    }
+    if (g_arena) return arena_alloc(p, bytes);
     hipError_t e = hipMalloc(p, bytes);
     if (e == hipSuccess) ptrs.push_back(*p);
     return e;
@@ -89,6 +95,7 @@ void cache_release(int dev) {
 void cache_release_device(int dev) { cache_release(dev); }
 
 hipError_t dev_malloc(void **p, size_t bytes) {
+    if (g_arena) return arena_alloc(p, bytes);
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
@@ -118,7 +125,7 @@ hipError_t dev_malloc(void **p, size_t bytes) {
 }
 
 void dev_free(void *p) {
-    if (!p) return;
+    if (!p || in_arena(p)) return;
     {
         std::lock_guard<std::mutex> lk(g_cache_mu);
         auto it = g_cache_live.find(p);
@@ -1451,6 +1458,169 @@ hegj_status hegj_selfjoin(hegj_index *h, const hegj_join_opts *opts_in, hegj_res
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     HEGJ_TRY(mark_done(r.get(), s));
     *out = r.release();
+    return HEGJ_OK;
+}
+
+namespace {
+std::mutex g_noalloc_mu[64];  // one per device: calls on a device run one at a time
+
+struct ArenaGuard {
+    explicit ArenaGuard(Arena *a) { set_thread_arena(a); }
+    ~ArenaGuard() { set_thread_arena(nullptr); }
+};
+}  // namespace
+
+hegj_status hegj_selfjoin_noalloc(hegj_index *h, const hegj_join_opts *opts_in, uint32_t *neighbors,
+                                  uint64_t capacity, uint64_t *row_off, uint32_t *row_cnt,
+                                  hegj_result_info *info) {
+    if (!h || !neighbors || !row_off || !row_cnt || !info || capacity == 0)
+        return fail(HEGJ_ERR_INVALID_ARG, "NULL argument or zero capacity");
+    if (h->device < 0 || h->device >= 64) return fail(HEGJ_ERR_INVALID_ARG, "device ordinal out of range");
+    const auto t_start = std::chrono::steady_clock::now();
+    HEGJ_TRY(set_device(h->device));
+    hegj_join_opts opts;
+    if (opts_in) {
+        opts = *opts_in;
+    } else {
+        hegj_join_opts_tuned(&opts, h->g.dim);
+        if (opts.refine == HEGJ_REFINE_MFMA) opts.refine = HEGJ_REFINE_AUTO;  // the grid kernels
+    }
+    if (opts.refine == HEGJ_REFINE_MFMA)
+        return fail(HEGJ_ERR_UNSUPPORTED, "hegj_selfjoin_noalloc runs the grid join kernels, not the MFMA refine");
+    if (opts.refine > HEGJ_REFINE_F32) return fail(HEGJ_ERR_INVALID_ARG, "unknown refine");
+    const uint32_t qb = opts.queries_per_unit ? opts.queries_per_unit : default_qb(h->g.dim);
+    if (qb != 4 && qb != 8 && qb != 16) return fail(HEGJ_ERR_INVALID_ARG, "queries_per_unit must be 4, 8 or 16");
+    // the index's lazily built arrays are inputs here: they must exist
+    const char *prep = "the index must have run hegj_selfjoin once with these options first";
+    if (!h->have_workload) return fail(HEGJ_ERR_INVALID_ARG, std::string(prep) + " (workload order)");
+    const uint32_t *order = nullptr;
+    if (opts.unit_order == HEGJ_ORDER_WORKLOAD) {
+        order = h->d_order;
+    } else if (opts.unit_order == HEGJ_ORDER_MORTON) {
+        if (!h->d_morton) return fail(HEGJ_ERR_INVALID_ARG, std::string(prep) + " (Morton order)");
+        if (h->morton_ok) order = h->d_morton;
+    } else if (opts.unit_order != HEGJ_ORDER_SPATIAL) {
+        return fail(HEGJ_ERR_INVALID_ARG, "unknown unit_order");
+    }
+    bool f32 = false;
+    float tacc = 0, trej = 0;
+    if (opts.refine == HEGJ_REFINE_F32 || opts.refine == HEGJ_REFINE_AUTO) {
+        bool ok = f32_supported(h->g.dim, qb) && h->soa32;
+        if (ok) {
+            if (h->g.f32dt)
+                tacc = trej = (float)h->g.thresh;
+            else
+                ok = f32_thresholds(h->g.thresh, h->g.dim, h->f32_radius, &tacc, &trej);
+        }
+        if (!ok && opts.refine == HEGJ_REFINE_F32)
+            return fail(HEGJ_ERR_UNSUPPORTED, f32_supported(h->g.dim, qb)
+                                                  ? std::string(prep) + " (FP32 copy)"
+                                                  : std::string("FP32 filter not built for this dim / queries_per_unit"));
+        f32 = ok;
+    }
+    hipStream_t s = h->stream;
+    const uint32_t n = h->g.n;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu[h->device]);
+    void *base = nullptr;
+    HEGJ_TRY(noalloc_scratch(h->device, &base));
+    Arena arena;
+    arena.base = static_cast<char *>(base);
+    arena.cap = kNoAllocScratchBytes;
+    ArenaGuard guard(&arena);  // every temporary below comes from the array
+    // declared after the guard, so its destructor (which hands its buffers
+    // back: a no-op for arena memory) runs while the arena is installed
+    JoinPlan P;
+    P.f32 = f32;
+    P.tacc32 = tacc;
+    P.trej32 = trej;
+    auto too_small = [&](hegj_status st) {
+        if (st != HEGJ_OK && arena.used + (1u << 20) >= arena.cap)
+            return fail(HEGJ_ERR_CAPACITY, "hegj_selfjoin_noalloc: this index needs more than the " +
+                                               std::to_string(kNoAllocScratchBytes >> 20) +
+                                               " MiB of fixed scratch (n = " + std::to_string(n) +
+                                               ", cells = " + std::to_string(h->ix.C) + ")");
+        return st;
+    };
+    // keep the plan's pointers out of the caller's way: JoinPlan frees them
+    // (a no-op for arena memory) before the guard uninstalls the arena
+    P.stream = s;
+    HIP_TRY(dev_malloc_n(&P.d_ctr, 1));
+    HIP_TRY(dev_malloc_n(&P.d_ctr2, 1));
+    HEGJ_TRY(zero_words(P.d_ctr, sizeof(Counters), s));
+    HEGJ_TRY(zero_words(row_off, sizeof(uint64_t) * n, s));
+    HEGJ_TRY(zero_words(row_cnt, sizeof(uint32_t) * n, s));
+    HEGJ_TRY(too_small(build_units(h->g, h->ix, qb, order, &P.units, &P.U, s)));
+    const uint32_t U = P.U;
+    if (arena_alloc(reinterpret_cast<void **>(&P.deferred), sizeof(uint32_t) * ((size_t)U + 1)) != hipSuccess)
+        return too_small(HEGJ_ERR_OUT_OF_MEMORY);
+    HEGJ_TRY(join_occupancy(h->g.dim, qb, h->device, &P.max_blocks, h->g.f32dt && !P.f32));
+    P.gmask_slots = 0;  // units whose masks exceed LDS re-evaluate in the write pass
+    P.parts = opts.xcd_parts ? std::min<uint32_t>(opts.xcd_parts, kMaxParts) : kMaxParts;
+    if (U < P.parts * 256) P.parts = 1;
+    HEGJ_TRY(too_small(split_units(h->ix, P.units, U, qb, h->d_work, P.parts, P.part_end, s)));
+    JoinArgs a = base_args(h, opts, P);
+    a.row_off = row_off;
+    a.row_cnt = row_cnt;
+    a.units = P.units;
+    a.n_units = U;
+    a.units_per_grab = opts.units_per_grab ? opts.units_per_grab : grab_for(P, U);
+    set_parts(a, P.parts, P.part_end);
+    a.out = neighbors;
+    a.out_base = 0;
+    a.capacity = capacity;
+    a.count_only = 0;
+    double join_ms = 0;  // host clock: no events are created here
+    Counters c{};
+    if (U > 0) {
+        HIP_TRY(hipStreamSynchronize(s));
+        const auto tj = std::chrono::steady_clock::now();
+        HEGJ_TRY(launch_join(a, qb, blocks_for(P, U, a.units_per_grab), s));
+        HIP_TRY(hipStreamSynchronize(s));
+        join_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tj).count();
+        HEGJ_TRY(launch_pair_hash(a, capacity, s));
+        HIP_TRY(hipMemcpyAsync(&c, P.d_ctr, sizeof(c), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    std::memset(info, 0, sizeof(*info));
+    info->total_candidates = h->total_candidates;
+    info->n_queries = n - h->n_dead;
+    info->n_units = U;
+    info->join_ms = join_ms;
+    info->join_launches = U > 0 ? 1 : 0;
+    info->join_kernel_avg_ms = join_ms;
+    info->workload_ms = h->workload_ms;
+    info->refine_used = P.f32 ? HEGJ_REFINE_F32 : HEGJ_REFINE_FP64;
+    if (c.mismatch)
+        return fail(HEGJ_ERR_HIP, "join: the write pass of " + std::to_string(c.mismatch) +
+                                      " work units disagreed with their counts");
+    if (c.n_deferred) {
+        // the rows do not fit: count them all (count-only pass) for the caller
+        JoinArgs e = a;
+        e.count_only = 1;
+        e.queue_heads = P.d_ctr2->queue_heads;
+        e.n_deferred = &P.d_ctr2->n_deferred;
+        e.cursor = &P.d_ctr2->cursor;
+        e.hash = &P.d_ctr2->hash;
+        e.evals = &P.d_ctr2->evals;
+        e.pairs = &P.d_ctr2->pairs;
+        HEGJ_TRY(zero_words(P.d_ctr2, sizeof(Counters), s));
+        HEGJ_TRY(launch_join(e, qb, blocks_for(P, U, e.units_per_grab), s));
+        Counters c2;
+        HIP_TRY(hipMemcpyAsync(&c2, P.d_ctr2, sizeof(c2), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        info->total_neighbors = c2.pairs;
+        return fail(HEGJ_ERR_CAPACITY, "hegj_selfjoin_noalloc: the result has " + std::to_string(c2.pairs) +
+                                           " neighbours, the buffer holds " + std::to_string(capacity) +
+                                           " (info->total_neighbors has the count)");
+    }
+    info->total_neighbors = c.pairs;
+    info->pair_hash = c.hash;
+    info->evals_executed = c.evals;
+    info->band_rechecks = c.band;
+    info->n_batches = 1;
+    info->estimated_pairs = 0;
+    info->total_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     return HEGJ_OK;
 }
 

@@ -76,13 +76,19 @@ struct DevIndex {
 };
 
 // Device bump allocator owned by one host thread.  While installed for that
-// thread, Scratch carves temporaries from it instead of calling hipMalloc /
-// hipFree (which synchronise the whole device); the owner resets `used`.
+// thread, Scratch and dev_malloc carve temporaries from it instead of calling
+// hipMalloc / hipFree (which synchronise the whole device), and dev_free of
+// such a pointer does nothing; the owner resets `used`.
 struct Arena {
     char *base = nullptr;
     size_t cap = 0, used = 0;
 };
 void set_thread_arena(Arena *a);  // nullptr: back to hipMalloc
+
+// The static device scratch of hegj_selfjoin_noalloc (hegj_scratch.hip):
+// its address on `device`, which must be the current device.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+hegj_status noalloc_scratch(int device, void **ptr);
 
 // Scratch allocator for temporaries of one call (freed by the destructor).
 struct Scratch {

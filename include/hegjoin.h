@@ -264,6 +264,26 @@ typedef struct {
 } hegj_result_info;
 hegj_status hegj_result_get_info(const hegj_result *res, hegj_result_info *info);
 
+/* hegj_selfjoin for callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  It calls no hipMalloc / hipFree: the caller
+ * passes the outputs as device buffers on the index's device --
+ * neighbors[capacity] (rows back to back), row_off[n] (offset of each point's
+ * row in neighbors) and row_cnt[n] (its length), by original point id -- and
+ * every temporary comes from a fixed 64 MiB device array per GPU (about
+ * 12 bytes per work unit plus 8 per non-empty cell: enough for the 5M-point
+ * 6-D workload).  An index that needs more fails with HEGJ_ERR_CAPACITY; so
+ * does a result larger than `capacity`, with info->total_neighbors set to
+ * the count needed.  The index's lazily built arrays are inputs: run
+ * hegj_selfjoin once with the same options first (HEGJ_ERR_INVALID_ARG
+ * otherwise).  Grid kernels only (refine MFMA is refused).  The array is
+ * shared by every call on its GPU and keeps the last call's contents; calls
+ * on one GPU are serialised by a lock inside the library (two processes
+ * each have their own array). */
+hegj_status hegj_selfjoin_noalloc(hegj_index *idx, const hegj_join_opts *opts, uint32_t *neighbors,
+                                  uint64_t capacity, uint64_t *row_off, uint32_t *row_cnt,
+                                  hegj_result_info *info);
+
 /* ------------------------------------------------------------------------
  * Hybrid CPU + GPU self-join: algorithms 1, 2 and 3 of
  * `./main dataset epsilon dim algorithm [static_partitioning]`
