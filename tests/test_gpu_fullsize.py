@@ -68,3 +68,33 @@ def test_fullsize_pair_set_equals_oracle(H, indexes, key, refine):
         assert info["refine_used"] == refine
     assert info["total_neighbors"] == f["pairs"]
     assert "%016x" % info["pair_hash"] == f["pair_hash"]
+
+
+# the MFMA workloads at full size (bench.py CONFIGS): no CPU oracle finishes
+# them in test time, so the plain FP64 kernel (the reference's chain, pinned
+# against the oracle on every fixture) is the yardstick
+@pytest.mark.parametrize("key", ["18d", "90d"])
+def test_fullsize_mfma_equals_fp64_kernel(H, key):
+    """18-D 2M and 90-D 1M: the split-BF16 MFMA guard band with its FP64 band
+    recheck gives the plain FP64 kernel's pair set (count and pair hash),
+    with rechecks in the band.  Prints both kernels' times."""
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from bench import CONFIGS
+    c = CONFIGS[key]
+    print("case", key, c["n"], "x", c["dim"], flush=True)
+    pts = H.generate_expo(c["n"], c["dim"], seed=0)
+    with H.GridIndex(pts, c["eps"], indexed_dims=c["k"]) as ix:
+        got = {}
+        for refine in ("mfma", "fp64"):
+            t = ix.selfjoin(refine=refine)
+            got[refine] = t.info
+            t.close()
+            print(f"  {refine}: {got[refine]['total_neighbors']} pairs, hash "
+                  f"{got[refine]['pair_hash']:016x}, join kernel {got[refine]['join_kernel_avg_ms']:.1f} ms, "
+                  f"band rechecks {got[refine]['band_rechecks']}", flush=True)
+    m, f = got["mfma"], got["fp64"]
+    assert m["refine_used"] == "mfma" and f["refine_used"] == "fp64"
+    assert m["total_neighbors"] == f["total_neighbors"]
+    assert m["pair_hash"] == f["pair_hash"]
+    assert m["band_rechecks"] > 0
