@@ -173,6 +173,8 @@ _lib.hegj_result_get_hybrid_info.argtypes = [_vp, C.POINTER(_HybridInfo)]
 _lib.hegj_superego_rows.argtypes = [_vp, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32, _vp,
                                     C.c_uint32, _vp, C.POINTER(C.POINTER(C.c_uint32)),
                                     C.POINTER(C.c_uint64)]
+_lib.hegj_gb_operands.argtypes = [_vp, C.c_uint32, C.c_uint32, _vp, C.c_double, _vp, _vp, _vp,
+                                  C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
 _lib.hegj_free.argtypes = [_vp]
 _lib.hegj_free.restype = None
 _lib.hegj_static_partition_model.argtypes = [C.c_uint32, C.c_uint32, C.c_double]
@@ -285,6 +287,25 @@ def superego_rows(database, epsilon, qids=None, threads=0):
     offsets = np.zeros(nq + 1, np.uint64)
     np.cumsum(counts, out=offsets[1:])
     return offsets, nb
+
+
+def gb_operands(points, centre, coef_scale=1.0):
+    """The MFMA refine's operands for points (n, dim) centred at centre:
+    (A', B') as uint16 BF16 bit patterns (n, kp), the guards w (n,) float32,
+    kp and the guard coefficient used (host code shared with the device)."""
+    pts = np.ascontiguousarray(points, np.float64)
+    n, dim = pts.shape
+    c = np.ascontiguousarray(centre, np.float64)
+    kp = C.c_uint32()
+    coef = C.c_double()
+    _check(_lib.hegj_gb_operands(None, 0, dim, None, coef_scale, None, None, None, C.byref(kp),
+                                 C.byref(coef)))
+    A = np.zeros((n, kp.value), np.uint16)
+    B = np.zeros((n, kp.value), np.uint16)
+    w = np.zeros(n, np.float32)
+    _check(_lib.hegj_gb_operands(pts.ctypes.data, n, dim, c.ctypes.data, coef_scale, A.ctypes.data,
+                                 B.ctypes.data, w.ctypes.data, C.byref(kp), C.byref(coef)))
+    return A, B, w, kp.value, coef.value
 
 
 def static_partition_model(n, dim, epsilon):

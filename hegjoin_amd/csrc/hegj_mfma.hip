@@ -45,61 +45,67 @@ constexpr int gb_rt(int kb) { return kb <= 4 ? 4 : 2; }
 constexpr int kGbBlk = 256;          // pair-list slots reserved per wave at a time
 constexpr uint64_t kSentinel = ~0ull;
 
-__device__ __forceinline__ uint16_t bf16_bits_rne(float f) {
-    uint32_t u = __float_as_uint(f);
+// Bit moves between float and uint32 that compile for host and device.
+__host__ __device__ __forceinline__ uint32_t f2u(float f) {
+    uint32_t u;
+    __builtin_memcpy(&u, &f, 4);
+    return u;
+}
+__host__ __device__ __forceinline__ float u2f(uint32_t u) {
+    float f;
+    __builtin_memcpy(&f, &u, 4);
+    return f;
+}
+
+__host__ __device__ __forceinline__ uint16_t bf16_bits_rne(float f) {
+    uint32_t u = f2u(f);
     u += 0x7FFFu + ((u >> 16) & 1u);
     return (uint16_t)(u >> 16);
 }
-__device__ __forceinline__ float bf16_to_f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+__host__ __device__ __forceinline__ float bf16_to_f(uint16_t b) { return u2f((uint32_t)b << 16); }
 
 // Neighbouring FP32 values (finite inputs), by bit arithmetic.
-__device__ __forceinline__ float f32_next_up(float f) {
-    uint32_t u = __float_as_uint(f);
-    if (f == 0.0f) return __uint_as_float(1u);
-    return __uint_as_float((u >> 31) ? u - 1 : u + 1);
+__host__ __device__ __forceinline__ float f32_next_up(float f) {
+    uint32_t u = f2u(f);
+    if (f == 0.0f) return u2f(1u);
+    return u2f((u >> 31) ? u - 1 : u + 1);
 }
-__device__ __forceinline__ float f32_next_down(float f) {
-    uint32_t u = __float_as_uint(f);
-    if (f == 0.0f) return __uint_as_float(0x80000001u);
-    return __uint_as_float((u >> 31) ? u + 1 : u - 1);
+__host__ __device__ __forceinline__ float f32_next_down(float f) {
+    uint32_t u = f2u(f);
+    if (f == 0.0f) return u2f(0x80000001u);
+    return u2f((u >> 31) ? u + 1 : u - 1);
 }
 
 // FP64 -> FP32 rounded toward -inf.
-__device__ __forceinline__ float f32_down(double v) {
+__host__ __device__ __forceinline__ float f32_down(double v) {
     float f = (float)v;
     if ((double)f > v) f = f32_next_down(f);
     return f;
 }
 
 // FP32 -> BF16 rounded toward -inf (finite inputs).
-__device__ __forceinline__ uint16_t bf16_bits_down(float f) {
-    uint32_t u = __float_as_uint(f);
+__host__ __device__ __forceinline__ uint16_t bf16_bits_down(float f) {
+    uint32_t u = f2u(f);
     uint32_t t = u & 0xFFFF0000u;
     if ((u & 0xFFFFu) && (u >> 31)) t += 0x10000u;  // negative: away from zero
     return (uint16_t)(t >> 16);
 }
 
 // Split v (FP64) into BF16 hi + lo.
-__device__ __forceinline__ void split_bf16(double v, uint16_t &hi, uint16_t &lo) {
+__host__ __device__ __forceinline__ void split_bf16(double v, uint16_t &hi, uint16_t &lo) {
     hi = bf16_bits_rne((float)v);
     lo = bf16_bits_rne((float)(v - (double)bf16_to_f(hi)));
 }
 
-// Per point (cell order): A' and B' rows, and w.
-__global__ __launch_bounds__(256) void k_gb_prepare(const double *__restrict__ soa, uint32_t n,
-                                                    uint32_t dim, uint32_t kp,
-                                                    const double *__restrict__ mean, double Acoef,
-                                                    double ub2, uint16_t *__restrict__ Av,
-                                                    uint16_t *__restrict__ Bv,
-                                                    float *__restrict__ w) {
-    uint32_t p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= n) return;
+// One point's A' and B' rows (kp bf16 each) and its guard w, from its
+// coordinates x (stride `stride` between dims) and the centre.
+__host__ __device__ inline void gb_prepare_point(const double *x, uint64_t stride, uint32_t dim,
+                                                 uint32_t kp, const double *mean, double Acoef,
+                                                 uint16_t *A, uint16_t *B, float *w) {
     const uint32_t g = dim + 2;  // group length
-    uint16_t *A = Av + (uint64_t)p * kp;
-    uint16_t *B = Bv + (uint64_t)p * kp;
     double N = 0.0;
     for (uint32_t l = 0; l < dim; l++) {
-        double y = soa[(uint64_t)l * n + p] - mean[l];
+        double y = x[(uint64_t)l * stride] - mean[l];
         uint16_t hi, lo;
         split_bf16(y, hi, lo);
         double yt = (double)bf16_to_f(hi) + (double)bf16_to_f(lo);
@@ -143,7 +149,19 @@ __global__ __launch_bounds__(256) void k_gb_prepare(const double *__restrict__ s
     double wd = (N - nt) + 2.0 * delta;
     float wf = (float)wd;
     if ((double)wf < wd) wf = f32_next_up(wf);
-    w[p] = wf;
+    *w = wf;
+}
+
+// Per point (cell order): A' and B' rows, and w.
+__global__ __launch_bounds__(256) void k_gb_prepare(const double *__restrict__ soa, uint32_t n,
+                                                    uint32_t dim, uint32_t kp,
+                                                    const double *__restrict__ mean, double Acoef,
+                                                    uint16_t *__restrict__ Av,
+                                                    uint16_t *__restrict__ Bv,
+                                                    float *__restrict__ w) {
+    uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= n) return;
+    gb_prepare_point(soa + p, n, dim, kp, mean, Acoef, Av + (uint64_t)p * kp, Bv + (uint64_t)p * kp, w + p);
 }
 
 __global__ __launch_bounds__(256) void k_dim_sums(const double *__restrict__ soa, uint32_t n,
@@ -427,7 +445,7 @@ __global__ __launch_bounds__(64 * kGbWaves) void k_gb_join(GbArgs a) {
                             const uint64_t m = __ballot(sv);
                             if (sv)
                                 surv[wid][ns + lanes_below(m)] =
-                                    make_uint2((uint32_t)(j * 64 + lane), __float_as_uint(acc[rt][ct][j]));
+                                    make_uint2((uint32_t)(j * 64 + lane), f2u(acc[rt][ct][j]));
                             ns += (uint32_t)__popcll(m);
                         }
                         if (ns == 0) continue;
@@ -445,7 +463,7 @@ __global__ __launch_bounds__(64 * kGbWaves) void k_gb_join(GbArgs a) {
                             if (i < ns && q < nq && ch * kGbChunk + ci < M) {
                                 const uint32_t qp = qpos0 + q, cp = cpos[buf][ci];
                                 key = ((uint64_t)qp << 32) | cp;
-                                const double hi = (double)__uint_as_float(e.y) + (double)a.w[qp] +
+                                const double hi = (double)u2f(e.y) + (double)a.w[qp] +
                                                   (double)a.w[cp];
                                 acc_ok = hi <= Tacc;
                                 band_ok = !acc_ok;
@@ -622,9 +640,8 @@ hegj_status gb_prepare(const GridGeom &g, const DevIndex &ix, uint32_t kp, uint1
     HIP_TRY(hipMalloc(Av, sizeof(uint16_t) * (size_t)n * kp));
     HIP_TRY(hipMalloc(Bv, sizeof(uint16_t) * (size_t)n * kp));
     HIP_TRY(hipMalloc(w, sizeof(float) * n));
-    const double ub = std::ldexp(1.0, -8);
     k_gb_prepare<<<ceil_div(n, 256), 256, 0, s>>>(ix.soa, n, dim, kp, mean, gb_coefficient(dim, kp),
-                                                 ub * ub, *Av, *Bv, *w);
+                                                 *Av, *Bv, *w);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s));
     return HEGJ_OK;
@@ -762,3 +779,21 @@ hegj_status gb_join(const GridGeom &g, const DevIndex &ix, const uint2 *units, u
 }
 
 }  // namespace hegj
+
+extern "C" hegj_status hegj_gb_operands(const double *pts, uint32_t n, uint32_t dim, const double *centre,
+                                        double coef_scale, uint16_t *A, uint16_t *B, float *w, uint32_t *kp,
+                                        double *coef) {
+    using namespace hegj;
+    const uint32_t kb = gb_kblocks(dim);
+    if (kb == 0) return fail(HEGJ_ERR_UNSUPPORTED, "no MFMA refine for this dimensionality");
+    const uint32_t kpv = kb * 32;
+    const double c = gb_coefficient(dim, kpv) * coef_scale;
+    if (kp) *kp = kpv;
+    if (coef) *coef = c;
+    if (!A && !B && !w) return HEGJ_OK;
+    if (!pts || !centre || !A || !B || !w) return fail(HEGJ_ERR_INVALID_ARG, "NULL argument");
+    for (uint32_t p = 0; p < n; p++)
+        gb_prepare_point(pts + (uint64_t)p * dim, 1, dim, kpv, centre, c, A + (uint64_t)p * kpv,
+                         B + (uint64_t)p * kpv, w + p);
+    return HEGJ_OK;
+}

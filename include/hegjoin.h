@@ -237,6 +237,18 @@ hegj_status hegj_selfjoin(hegj_index *idx, const hegj_join_opts *opts, hegj_resu
  * epsilon^2.  (No reference counterpart; exposed for callers and tests.) */
 hegj_status hegj_f32_thresholds(double epsilon, uint32_t dim, double radius, float *tacc,
                                 float *trej);
+/* The operands of the MFMA refine (no reference counterpart; for callers and
+ * the band's tests): for n points (row-major n x dim) centred at centre[dim],
+ * the split-BF16 rows A'[n * kp] and B'[n * kp] (BF16 bit patterns) and the
+ * guard w[n], by the same code the device runs.  coef_scale multiplies the
+ * guard coefficient (1 = the library's).  *kp = padded K, *coef = the
+ * coefficient used; with A, B and w all NULL only those two are returned.
+ * A pair is rejected when its FP32 Gram value s' = A'_q . B'_c > T (rounded
+ * up to FP32), accepted when s' + w_q + w_c <= T (1 - 4 * 2^-53), and
+ * rechecked in FP64 otherwise.  Host only. */
+hegj_status hegj_gb_operands(const double *pts, uint32_t n, uint32_t dim, const double *centre,
+                             double coef_scale, uint16_t *A, uint16_t *B, float *w, uint32_t *kp,
+                             double *coef);
 hegj_status hegj_result_free(hegj_result *res);
 
 typedef struct {

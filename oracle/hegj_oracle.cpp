@@ -609,6 +609,17 @@ uint32_t og_index_build(const double *pts, uint32_t n, uint32_t dim, uint32_t k,
     return (uint32_t)g.cellKeys.size();
 }
 
+void og_chain(const double *q, const double *c, uint32_t npairs, uint32_t dim, double *acc) {
+    for (uint64_t i = 0; i < npairs; i++) {
+        double a = 0;
+        for (uint32_t l = 0; l < dim; l++) {
+            const double d = c[i * dim + l] - q[i * dim + l];
+            a = std::fma(d, d, a);
+        }
+        acc[i] = a;
+    }
+}
+
 double og_threshold(double eps) {
     double s = eps * eps;
     const double inf = std::numeric_limits<double>::infinity();

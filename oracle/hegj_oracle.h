@@ -77,6 +77,11 @@ uint32_t og_index_build(const double *pts, uint32_t n, uint32_t dim, uint32_t k,
                         const double *minArr, const uint32_t *nCells, uint32_t *indexLookupArr,
                         og_cell_lookup *lookup, og_grid *grid);
 
+/* evalPoint's distance (kernel.cu:347-355) for explicit pairs: acc[i] =
+ * fma chain over dims 0..dim-1 of (c_i - q_i)^2, q and c row-major
+ * npairs x dim. */
+void og_chain(const double *q, const double *c, uint32_t npairs, uint32_t dim, double *acc);
+
 /* S7: largest s with RN(sqrt(s)) <= eps (so sqrt(acc) <= eps <=> acc <= T). */
 double og_threshold(double eps);
 float og_threshold_f(float eps);

@@ -52,6 +52,8 @@ def lib():
         L.og_index_build.restype = u32
         L.og_threshold.argtypes = [f64]
         L.og_threshold.restype = f64
+        L.og_chain.argtypes = [_f64p, _f64p, u32, u32, _f64p]
+        L.og_chain.restype = None
         L.og_threshold_f.argtypes = [C.c_float]
         L.og_threshold_f.restype = C.c_float
         L.og_workload.argtypes = [_f64p, u32, u32, u32, f64, _f64p, _u32p, _u32p, C.c_void_p,
@@ -148,6 +150,15 @@ def index_build(pts, k, eps):
 
 def threshold(eps):
     return lib().og_threshold(eps)
+
+
+def chain(q, c):
+    """The reference's FP64 distance chain for pairs (rows of q and c)."""
+    q = np.ascontiguousarray(q, np.float64)
+    c = np.ascontiguousarray(c, np.float64)
+    out = np.zeros(len(q))
+    lib().og_chain(q, c, len(q), q.shape[1], out)
+    return out
 
 
 def threshold_f(eps):
