@@ -609,7 +609,21 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())
             const uint32_t nchunks = (M + 63) / 64;
             const uint32_t nmask = nq * nchunks;
             const bool keep = nmask <= a.gmask_slots || nmask <= (uint32_t)kMaskSlots;
-            uint64_t *const hs = nmask <= (uint32_t)kMaskSlots ? hs_lds : hs_glb;
+            // masks in LDS (ds_* instructions) unless the unit has more than
+            // fit; every access names its memory, so no generic (flat) access
+            // enters the loops: a flat store would count against both the
+            // vector-memory and the LDS counters, and the candidate ring's
+            // loads could then only be waited for all at once
+            const bool lds_masks = nmask <= (uint32_t)kMaskSlots;
+            auto mask_put = [&](uint32_t i, uint64_t v) __attribute__((always_inline)) {
+                if (lds_masks)
+                    hs_lds[i] = v;
+                else
+                    gst(hs_glb + i, v);
+            };
+            auto mask_get = [&](uint32_t i) __attribute__((always_inline)) {
+                return lds_masks ? hs_lds[i] : gld(hs_glb + i);
+            };
             uint32_t cnt_l = 0;  // lane qi: hits of query qi so far
             uint64_t mine = 0;   // lane qi: query qi's mask of the current chunk
             uint32_t nband = 0;  // FP32 filter: (query, chunk) pairs with lanes in the band
@@ -672,6 +686,15 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())
                                 }
                             }
                             {
+                                // refill the ring slot only after the distances
+                                // that read it: the new values then take the same
+                                // registers and the loop carries the ring without
+                                // copies (a copy of a value still loading forces a
+                                // wait for every load in flight)
+                                if constexpr (QL > 1) {
+#pragma unroll
+                                    for (int qi = 0; qi < QL; qi++) __asm__ volatile("" ::"v"(accf[qi]) : "memory");
+                                }
                                 const uint32_t tn = (ch + kDepth) * 64 + lane;
                                 load_cand32<DR>(a, tn < M ? cand_pos_adv(rl, nslots, tn, slot) : 0u, ringf[p]);
                             }
@@ -718,7 +741,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())
                                                               ((uint32_t)qi < nq ? vmask : 0ull), qi);
                         }
                         cnt_l += (uint32_t)__popcll(mine);
-                        if (keep && (uint32_t)lane < nq && ch < nchunks) hs[ch * nq + lane] = mine;
+                        if (keep && (uint32_t)lane < nq && ch < nchunks) mask_put(ch * nq + lane, mine);
                     }
                 }
             };
@@ -774,7 +797,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())
                         }
                         const uint64_t madd = __ballot(hit);
                         if (madd) {
-                            if (keep && lane == 0) hs[ch * nq + qi] |= madd;
+                            if (keep && lane == 0) mask_put(ch * nq + qi, mask_get(ch * nq + qi) | madd);
                             cnt_l += (uint32_t)lane == qi ? (uint32_t)__popcll(madd) : 0u;
                         }
                     }
@@ -840,7 +863,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())
                 auto fetch = [&](uint32_t c, uint32_t &id, uint64_t &mk) {
                     const uint32_t t = c * 64 + lane;
                     id = gld(a.ix.ila + (t < M ? cand_pos_adv(rl, nslots, t, slot2) : 0u));
-                    mk = (c < nchunks && (uint32_t)lane < nq) ? hs[c * nq + lane] : 0ull;
+                    mk = (c < nchunks && (uint32_t)lane < nq) ? mask_get(c * nq + lane) : 0ull;
                 };
 #pragma unroll
                 for (int p = 0; p < kW; p++) fetch((uint32_t)p, idr[p], mkr[p]);
