@@ -472,7 +472,7 @@ hegj_status make_plan(hegj_index *h, const hegj_join_opts &opts, uint32_t qb, co
                       JoinPlan &P, bool all_cells = false) {
     hipStream_t s = h->stream;
     if (opts.refine == HEGJ_REFINE_F32 || opts.refine == HEGJ_REFINE_AUTO) {
-        bool ok = f32_supported(h->g.dim, qb);
+        bool ok = f32_supported(h->g.dim, qb, h->g.n);
         if (ok) {
             HEGJ_TRY(ensure_soa32(h));
             if (h->g.f32dt)  // the float chain itself: one threshold, no band
@@ -483,7 +483,7 @@ hegj_status make_plan(hegj_index *h, const hegj_join_opts &opts, uint32_t qb, co
         if (!ok && opts.refine == HEGJ_REFINE_F32)
             return fail(HEGJ_ERR_UNSUPPORTED,
                         "FP32 filter: built for dim 2 (4/8/16 queries per unit) and 6 (4/8), "
-                        "and needs a band below 1% of eps^2");
+                        "up to 2^32 bytes of FP32 coordinates, and needs a band below 1% of eps^2");
         P.f32 = ok;
     }
     P.stream = s;
@@ -493,7 +493,8 @@ hegj_status make_plan(hegj_index *h, const hegj_join_opts &opts, uint32_t qb, co
     HIP_TRY(dev_malloc_n(&P.deferred, (size_t)U + 1));
     HIP_TRY(dev_malloc_n(&P.d_ctr, 1));
     HIP_TRY(dev_malloc_n(&P.d_ctr2, 1));
-    HEGJ_TRY(join_occupancy(h->g.dim, qb, h->device, &P.max_blocks, h->g.f32dt && !P.f32));
+    HEGJ_TRY(join_occupancy(h->g.dim, qb, h->device, &P.max_blocks,
+                            P.f32 ? JoinKind::f32 : h->g.f32dt ? JoinKind::flt : JoinKind::fp64));
     // per-wave hit-mask scratch for units whose masks exceed LDS
     HIP_TRY(dev_malloc_n(&P.gmask, (size_t)P.max_blocks * 4 * P.gmask_slots));
     // unit ranges per XCD, split by estimated work
@@ -1505,7 +1506,7 @@ hegj_status hegj_selfjoin_noalloc(hegj_index *h, const hegj_join_opts *opts_in, 
     bool f32 = false;
     float tacc = 0, trej = 0;
     if (opts.refine == HEGJ_REFINE_F32 || opts.refine == HEGJ_REFINE_AUTO) {
-        bool ok = f32_supported(h->g.dim, qb) && h->soa32;
+        bool ok = f32_supported(h->g.dim, qb, h->g.n) && h->soa32;
         if (ok) {
             if (h->g.f32dt)
                 tacc = trej = (float)h->g.thresh;
@@ -1513,7 +1514,7 @@ hegj_status hegj_selfjoin_noalloc(hegj_index *h, const hegj_join_opts *opts_in, 
                 ok = f32_thresholds(h->g.thresh, h->g.dim, h->f32_radius, &tacc, &trej);
         }
         if (!ok && opts.refine == HEGJ_REFINE_F32)
-            return fail(HEGJ_ERR_UNSUPPORTED, f32_supported(h->g.dim, qb)
+            return fail(HEGJ_ERR_UNSUPPORTED, f32_supported(h->g.dim, qb, h->g.n)
                                                   ? std::string(prep) + " (FP32 copy)"
                                                   : std::string("FP32 filter not built for this dim / queries_per_unit"));
         f32 = ok;
@@ -1553,7 +1554,8 @@ hegj_status hegj_selfjoin_noalloc(hegj_index *h, const hegj_join_opts *opts_in, 
     const uint32_t U = P.U;
     if (arena_alloc(reinterpret_cast<void **>(&P.deferred), sizeof(uint32_t) * ((size_t)U + 1)) != hipSuccess)
         return too_small(HEGJ_ERR_OUT_OF_MEMORY);
-    HEGJ_TRY(join_occupancy(h->g.dim, qb, h->device, &P.max_blocks, h->g.f32dt && !P.f32));
+    HEGJ_TRY(join_occupancy(h->g.dim, qb, h->device, &P.max_blocks,
+                            P.f32 ? JoinKind::f32 : h->g.f32dt ? JoinKind::flt : JoinKind::fp64));
     P.gmask_slots = 0;  // units whose masks exceed LDS re-evaluate in the write pass
     P.parts = opts.xcd_parts ? std::min<uint32_t>(opts.xcd_parts, kMaxParts) : kMaxParts;
     if (U < P.parts * 256) P.parts = 1;

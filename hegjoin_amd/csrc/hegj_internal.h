@@ -39,6 +39,7 @@ constexpr int kMaxK = HEGJ_MAX_INDEXED_DIMS;
 constexpr int kMaxRows = 243;      // 3^(kMaxK-1) stencil rows
 constexpr int kMaxBatches = 64;    // result batches per join
 constexpr uint32_t kAosMaxDim = 8; // point-major copies of the coordinates up to this dim
+constexpr uint32_t kAos32Pad = 512; // +inf points after the n of the point-major FP32 copy
 
 // Grid geometry passed by value to kernels.
 struct GridGeom {
@@ -202,13 +203,16 @@ hegj_status index_f32(hegj_index *h, const float **soa32, const float **aos32, d
                       const double **centre);
 // FP32-filter thresholds for threshold T (false: band wider than 1% of T).
 bool f32_thresholds(double T, uint32_t dim, double R, float *tacc, float *trej);
-// Whether the FP32 filter is instantiated for (dim, qb).
-bool f32_supported(uint32_t dim, uint32_t qb);
+// Whether the FP32 filter is instantiated for (dim, qb) and its 32-bit
+// offsets cover n points.
+bool f32_supported(uint32_t dim, uint32_t qb, uint32_t n);
 // Zero `bytes` (a multiple of 4) with a kernel (safe inside graph capture).
 hegj_status zero_words(void *p, size_t bytes, hipStream_t s);
-// Workgroups of the persistent join kernel that fit on the device at once.
-// flt: the DTYPE=float generic instance.
-hegj_status join_occupancy(uint32_t dim, uint32_t qb, int device, uint32_t *blocks, bool flt = false);
+// Workgroups of the persistent join kernel that fit on the device at once;
+// kind: the FP64 kernel, the DTYPE=float generic instance, or the FP32 filter.
+enum class JoinKind { fp64, flt, f32 };
+hegj_status join_occupancy(uint32_t dim, uint32_t qb, int device, uint32_t *blocks,
+                           JoinKind kind = JoinKind::fp64);
 
 // ---- MFMA guard-band refine (hegj_mfma.hip) --------------------------------
 struct GbOutput {

@@ -29,6 +29,7 @@
 // The per-query (offset, count) table is the reference's neighborTable
 // (GPU.cu:1731-1776) with a single data array per batch.
 #include <type_traits>
+#include <utility>
 
 #include "hegj_internal.h"
 
@@ -46,8 +47,8 @@ constexpr uint32_t kMaxQueryLds = 36u << 10;
 // Compacted stencil-row list of one unit in LDS.
 constexpr int kBandList = 64;  // FP32 filter: (query, chunk) pairs awaiting the FP64 recheck
 struct RowList {
-    uint32_t start[kMaxRows];    // first candidate position of slot s
-    uint32_t off[kMaxRows + 1];  // prefix of slot lengths
+    uint32_t start[kMaxRows + 1];  // first candidate position of slot s (+1: the FP32 kernel's padding slot)
+    uint32_t off[kMaxRows + 1];    // prefix of slot lengths
     float2 gap[16 * kMaxK];      // per (query, indexed dim): squared gaps to the -1 / +1 cell
     uint32_t band[kBandList];    // (chunk << 4) | query
 };
@@ -344,13 +345,16 @@ __device__ __forceinline__ void load_cand32(const JoinArgs &a, uint32_t pos, flo
     }
 }
 
-// v with lane `lane` replaced by the wave-uniform s (v_writelane_b32; the
-// lane select comes from a scalar move, never from a VALU write).
+// llvm.amdgcn.writelane (v_writelane_b32): `old` with lane `lane` replaced by
+// the wave-uniform `src`.  Called as the intrinsic, not as inline asm, so
+// the compiler sees the instruction and inserts the wait states it needs
+// after a VALU write of `src` (v_cmp, v_readfirstlane); inline asm hides the
+// read and can pick up the register's previous value.
+extern "C" __device__ int hegj_writelane_i32(int src, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
+
+// v with lane `lane` replaced by the wave-uniform s.
 __device__ __forceinline__ uint32_t set_lane_u32(uint32_t v, uint32_t s, uint32_t lane) {
-    // s is wave-uniform; readfirstlane pins it to a scalar register
-    const uint32_t su = (uint32_t)__builtin_amdgcn_readfirstlane((int)s);
-    __asm__("v_writelane_b32 %0, %1, m0" : "+v"(v) : "s"(su), "{m0}"(lane));
-    return v;
+    return (uint32_t)hegj_writelane_i32((int)s, (int)lane, (int)v);
 }
 __device__ __forceinline__ uint64_t set_lane_u64(uint64_t v, uint64_t s, int lane) {
     const uint32_t lo = set_lane_u32((uint32_t)v, (uint32_t)s, (uint32_t)lane);
@@ -965,6 +969,481 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())
     }
 }
 
+// ---------------------------------------------------------------------------
+// k_join_f32: the FP32-filter join for 2-D and 6-D
+// ---------------------------------------------------------------------------
+// Same units, stencil rows, queue, thresholds and result layout as k_join
+// with the FP32 filter; the candidate loop is built so that a chunk of 64
+// candidates costs little beyond the distance arithmetic:
+//  - Lane positions come from a chunk table instead of a per-lane search.
+//    For each chunk of a window of kTab chunks the table holds S, the slot of
+//    the chunk's lane 0, and B, a bitmap whose bit j says that a slot starts
+//    at lane j + 1.  Lane l's slot is S + popcount(B & lanes below l) (one
+//    v_mbcnt pair) and its byte offset one LDS read away.
+//  - Padding: the FP32 copy ends with kAos32Pad points at +inf, and one more
+//    slot after the unit's last maps the lanes past its last candidate onto
+//    them; padding queries are +inf too.  Such lanes are neither accepted nor
+//    in the band, so the loop applies no validity masks.
+//  - Band detection is a scalar OR per query; only the rare chunks with band
+//    lanes list their (query, chunk) pairs for the FP64 recheck.
+//  - A bitmap of the chunks with hits lets the write phase skip the others.
+constexpr int kRing = 4;                // candidate chunks in flight per wave
+constexpr int kTab = 64;                // chunk-table entries (one per lane)
+constexpr int kTabStep = kTab - kRing;  // chunks per table window: the ring's prefetches stay inside it
+constexpr int kHitWords = 16;           // hit bitmap: units of up to 1024 chunks
+struct F32Lds {
+    uint4 tab[kTab];               // {B low, B high, S, 0} of chunk c0 + i
+    uint32_t bb[kMaxRows + 1];     // slot s: byte offset of candidate t is bb[s] + t * stride (mod 2^32)
+    uint64_t hitw[kHitWords];      // bit c: chunk c has a hit
+};
+
+// Orders this wave's LDS accesses (one wave's LDS instructions execute in
+// order, so the compiler's ordering is all that is needed).
+__device__ __forceinline__ void wave_lds_order() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Chunk table for chunks c0 .. c0 + kTab - 1.  Slot s (1 <= s <= nslots; slot
+// nslots is the padding slot, off[nslots] = M) starts at flattened candidate
+// off[s] and sets bit (off[s] - 1) % 64 of chunk (off[s] - 1) / 64.  S of
+// chunk c is the number of slots s >= 1 with off[s] <= 64 c: the slot of its
+// lane 0 (off[0] = 0 and the offsets increase strictly).
+__device__ __forceinline__ void build_tab(F32Lds &f, const RowList &rl, uint32_t nslots, uint32_t c0) {
+    const int lane = threadIdx.x & 63;
+    wave_lds_order();
+    f.tab[lane] = make_uint4(0u, 0u, 0u, 0u);
+    wave_lds_order();
+    const uint32_t t0 = c0 * 64u, tend = t0 + 64u * kTab;
+    uint32_t before = 0;
+    for (uint32_t s0 = 1; s0 <= nslots; s0 += 64) {
+        const uint32_t s = s0 + (uint32_t)lane;
+        const bool in = s <= nslots;
+        const uint32_t o = in ? rl.off[s] : 0u;
+        before += (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(in && o <= t0));
+        if (in && o > t0 && o <= tend) {
+            const uint32_t j = o - 1u - t0;
+            atomicOr(reinterpret_cast<unsigned long long *>(&f.tab[j >> 6]), 1ull << (j & 63u));
+        }
+    }
+    wave_lds_order();
+    const uint4 e = f.tab[lane];
+    const uint32_t pc = (uint32_t)__popc(e.x) + (uint32_t)__popc(e.y);
+    uint32_t x = pc;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    f.tab[lane].z = before + x - pc;
+    wave_lds_order();
+}
+
+// Slot of the calling lane in chunk c (c0 <= c < c0 + kTab).
+__device__ __forceinline__ uint32_t tab_slot(const F32Lds &f, uint32_t c, uint32_t c0) {
+    const uint4 e = f.tab[c - c0];
+    return __builtin_amdgcn_mbcnt_hi(e.y, __builtin_amdgcn_mbcnt_lo(e.x, e.z));
+}
+
+// v with lane L replaced by the wave-uniform s (the lane select is an
+// immediate: no scalar move per write).
+template <int L>
+__device__ __forceinline__ uint64_t put_lane(uint64_t v, uint64_t s) {
+    const uint32_t lo = (uint32_t)hegj_writelane_i32((int)(uint32_t)s, L, (int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)hegj_writelane_i32((int)(uint32_t)(s >> 32), L, (int)(uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// f(std::integral_constant<int, i>) for i = 0 .. N - 1.
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_seq(F &&f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+    static_for_seq(f, std::make_integer_sequence<int, N>{});
+}
+
+// D floats of the point-major FP32 copy at a 32-bit byte offset.
+template <int D>
+__device__ __forceinline__ void load_cand32_off(const float *aos32, uint32_t byte_off, float *x) {
+    typedef float gf2 __attribute__((ext_vector_type(2)));
+    const gf2 *base = reinterpret_cast<const gf2 *>(aos32);
+#pragma unroll
+    for (int l = 0; l < D / 2; l++) {
+        const gf2 v = gload_off(base, byte_off + 8u * (uint32_t)l);
+        x[2 * l] = v.x;
+        x[2 * l + 1] = v.y;
+    }
+}
+
+template <int D, int QB, bool IND>
+__global__ __launch_bounds__(64 * kWavesPerBlock, 4) void k_join_f32(JoinArgs av, const JoinArgs *__restrict__ ap) {
+    const JoinArgs &a = IND ? *ap : av;
+    static_assert((D == 2 || D == 6) && QB * D <= 48 && QB <= 16, "FP32 filter: 2-D and 6-D");
+    __shared__ RowList rls[kWavesPerBlock];
+    __shared__ HitStore hs_all[kWavesPerBlock];
+    __shared__ F32Lds fls[kWavesPerBlock];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    RowList &rl = rls[wid];
+    F32Lds &fl = fls[wid];
+    uint64_t *const hs_lds = hs_all[wid].w;
+    uint64_t *const hs_glb = a.gmask + (uint64_t)(blockIdx.x * kWavesPerBlock + wid) * a.gmask_slots;
+    constexpr uint32_t kStride = D * (uint32_t)sizeof(float);
+    constexpr uint32_t kChunkBytes = 64u * kStride;
+    const uint32_t lane_off = (uint32_t)lane * kStride;
+    const double T = a.g.thresh;
+    const float tacc = a.tacc32, trej = a.trej32;
+    uint64_t my_evals = 0, my_pairs = 0, my_band = 0;
+    float qregf[QB][D];
+    uint32_t part = (a.part_by_xcc ? xcc_id() : blockIdx.x) % a.n_parts;
+    uint32_t tries = 0, grab = 0, gend = 0;
+
+    while (next_grab(a, part, tries, grab, gend)) {
+        for (uint32_t ui = grab; ui < gend; ui++) {
+            const uint2 unit = a.units[ui];
+            const uint32_t cell = wave_bcast_u32(unit.x), qoff = wave_bcast_u32(unit.y);
+            const uint32_t cstart = a.ix.cell_start[cell];
+            const uint32_t csize = a.ix.cell_start[cell + 1] - cstart;
+            const uint32_t nq = (uint32_t)__builtin_amdgcn_readfirstlane((int)min((uint32_t)QB, csize - qoff));
+            const uint32_t qpos0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)(cstart + qoff));
+            uint32_t nslots;
+            bool live;
+            const uint32_t M = (uint32_t)__builtin_amdgcn_readfirstlane(
+                (int)build_rows<QB>(a, cell, rl, &nslots, &live, qpos0, nq));
+            nslots = (uint32_t)__builtin_amdgcn_readfirstlane((int)nslots);
+            // the unit's query coordinates, once (lane i loads element i);
+            // padding queries are +inf
+            {
+                constexpr int NQD = QB * D;
+                const int qi = lane / D, l = lane % D;
+                float v = __builtin_inff();
+                if (lane < NQD && (uint32_t)qi < nq) v = a.soa32[(uint64_t)l * a.g.n + qpos0 + qi];
+#pragma unroll
+                for (int j = 0; j < NQD; j++)
+                    qregf[j / D][j % D] = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(v), j));
+            }
+            const uint32_t nchunks = (M + 63) / 64;
+            const uint32_t nmask = nq * nchunks;
+            const bool keep = nmask <= a.gmask_slots || nmask <= (uint32_t)kMaskSlots;
+            const bool lds_masks = nmask <= (uint32_t)kMaskSlots;
+            auto mask_put = [&](uint32_t i, uint64_t v) __attribute__((always_inline)) {
+                if (lds_masks)
+                    hs_lds[i] = v;
+                else
+                    gst(hs_glb + i, v);
+            };
+            auto mask_get = [&](uint32_t i) __attribute__((always_inline)) {
+                return lds_masks ? hs_lds[i] : gld(hs_glb + i);
+            };
+            const bool hit_bits = nchunks <= 64u * (uint32_t)kHitWords;
+            uint32_t cnt_l = 0;  // lane qi: hits of query qi so far
+            uint64_t mine = 0;   // lane qi: query qi's mask of the current chunk
+            uint32_t nband = 0;  // (query, chunk) pairs with lanes in the band
+            uint32_t tab_c0 = 0; // window of the chunk table
+            wave_lds_order();
+            // slot byte bases; the padding slot (index nslots, off = M) maps
+            // candidate t >= M to padding point n + t - M
+            for (uint32_t s = (uint32_t)lane; s <= nslots; s += 64)
+                fl.bb[s] = s < nslots ? (rl.start[s] - rl.off[s]) * kStride : (a.g.n - M) * kStride;
+
+            // The loop is built for QL queries (QB, QB / 2 or 1: cells of
+            // one or two points are common); the query registers and masks
+            // keep their QB layout.
+            auto chunk_loop = [&](auto qlc) __attribute__((always_inline)) {
+                constexpr int QL = decltype(qlc)::value;
+                float ring[kRing][D];
+                uint64_t hw = 0;  // hit bits of the current 64 chunks
+                for (uint32_t c0 = 0; c0 < nchunks; c0 += kTabStep) {
+                    build_tab(fl, rl, nslots, c0);
+                    tab_c0 = c0;
+                    if (c0 == 0) {
+#pragma unroll
+                        for (int p = 0; p < kRing; p++)
+                            load_cand32_off<D>(a.aos32, fl.bb[tab_slot(fl, p, 0)] + (uint32_t)p * kChunkBytes + lane_off,
+                                               ring[p]);
+                    }
+                    const uint32_t cend = min(c0 + (uint32_t)kTabStep, nchunks);
+                    for (uint32_t ch0 = c0; ch0 < cend; ch0 += kRing) {
+#pragma unroll
+                        for (int p = 0; p < kRing; p++) {
+                            const uint32_t ch = ch0 + p;
+                            float accf[QL];
+#pragma unroll
+                            for (int qi = 0; qi < QL; qi++) {
+                                accf[qi] = 0.0f;
+#pragma unroll
+                                for (int l = 0; l < D; l++) {
+                                    const float dd = ring[p][l] - qregf[qi][l];
+                                    accf[qi] = __builtin_fmaf(dd, dd, accf[qi]);
+                                }
+                            }
+                            {
+                                // refill the ring slot with chunk ch + kRing only
+                                // after the distances that read it (the loop then
+                                // carries the ring without register copies)
+                                if constexpr (QL > 1) {
+#pragma unroll
+                                    for (int qi = 0; qi < QL; qi++) __asm__ volatile("" ::"v"(accf[qi]) : "memory");
+                                }
+                                const uint32_t cn = ch + kRing;
+                                load_cand32_off<D>(a.aos32, fl.bb[tab_slot(fl, cn, c0)] + cn * kChunkBytes + lane_off,
+                                                   ring[p]);
+                            }
+                            uint64_t bandm = 0, anyh = 0;
+                            static_for<QL>([&](auto qc) __attribute__((always_inline)) {
+                                constexpr int qi = decltype(qc)::value;
+                                const uint64_t ma = __builtin_amdgcn_ballot_w64(accf[qi] <= tacc);
+                                const uint64_t mr = __builtin_amdgcn_ballot_w64(accf[qi] <= trej);
+                                bandm |= mr & ~ma;
+                                anyh |= ma;
+                                mine = put_lane<qi>(mine, ma);
+                            });
+                            if (bandm) {
+                                // decided after the loop by the exact FP64 chain
+#pragma unroll
+                                for (int qi = 0; qi < QL; qi++) {
+                                    const uint64_t mb = __builtin_amdgcn_ballot_w64(accf[qi] > tacc && accf[qi] <= trej);
+                                    if (mb) {
+                                        if (lane == 0 && nband < (uint32_t)kBandList) rl.band[nband] = (ch << 4) | (uint32_t)qi;
+                                        nband++;
+                                        my_band += (uint64_t)__popcll(mb);
+                                    }
+                                }
+                            }
+                            cnt_l += (uint32_t)__popcll(mine);
+                            if (ch < nchunks) {
+                                if (keep && (uint32_t)lane < nq) mask_put(ch * nq + lane, mine);
+                                hw |= (anyh ? 1ull : 0ull) << (ch & 63u);
+                            }
+                        }
+                        if (hit_bits && (((ch0 + kRing) & 63u) == 0 || ch0 + kRing >= nchunks)) {
+                            if (lane == 0) fl.hitw[ch0 >> 6] = hw;
+                            hw = 0;
+                        }
+                    }
+                }
+            };
+            if (nq == 1)
+                chunk_loop(std::integral_constant<int, 1>{});
+            else if (nq <= (uint32_t)QB / 2)
+                chunk_loop(std::integral_constant<int, QB / 2>{});
+            else
+                chunk_loop(std::integral_constant<int, QB>{});
+
+            // The band: the reference's exact FP64 chain decides the lanes
+            // whose FP32 sum lies between the two thresholds.  Listed
+            // (query, chunk) pairs, or all of them when the list overflowed.
+            if (nband) {
+                wave_lds_order();
+                const bool all = nband > (uint32_t)kBandList;
+                const uint32_t nitems = all ? nq * nchunks : nband;
+                const uint64_t n = a.g.n;
+                for (uint32_t e = 0; e < nitems; e++) {
+                    uint32_t qi, ch;
+                    if (all) {
+                        ch = e / nq;
+                        qi = e - ch * nq;
+                    } else {
+                        const uint32_t v = rl.band[e];
+                        qi = v & 15u;
+                        ch = v >> 4;
+                    }
+                    const uint32_t t = ch * 64 + lane;
+                    const bool valid = t < M;
+                    const uint32_t pos = valid ? cand_pos(rl, nslots, t) : 0u;
+                    float af = 0.0f;
+#pragma unroll
+                    for (int l = 0; l < D; l++) {
+                        const float dd = a.soa32[(uint64_t)l * n + pos] - a.soa32[(uint64_t)l * n + qpos0 + qi];
+                        af = __builtin_fmaf(dd, dd, af);
+                    }
+                    const bool band = valid && !(af <= tacc) && !(af > trej);
+                    bool hit = false;
+                    if (band) {
+                        double e64 = 0.0;
+#pragma unroll
+                        for (int l = 0; l < D; l++) {
+                            const double dd = a.ix.soa[(uint64_t)l * n + pos] - a.ix.soa[(uint64_t)l * n + qpos0 + qi];
+                            e64 = __builtin_fma(dd, dd, e64);
+                        }
+                        hit = e64 <= T;
+                    }
+                    const uint64_t madd = __builtin_amdgcn_ballot_w64(hit);
+                    if (madd) {
+                        if (lane == 0) {
+                            if (keep) mask_put(ch * nq + qi, mask_get(ch * nq + qi) | madd);
+                            if (hit_bits) fl.hitw[ch >> 6] |= 1ull << (ch & 63u);
+                        }
+                        cnt_l += (uint32_t)lane == qi ? (uint32_t)__popcll(madd) : 0u;
+                    }
+                }
+            }
+            my_evals += (uint64_t)M * nq;
+            // lane qi: first slot of query qi within the unit's segment
+            uint32_t incl_l = cnt_l;
+#pragma unroll
+            for (int o = 1; o < QB; o <<= 1) {
+                const uint32_t y = __shfl_up(incl_l, o, 64);
+                if (lane >= o) incl_l += y;
+            }
+            const uint32_t excl_l = incl_l - cnt_l;
+            const uint32_t total = __shfl(incl_l, QB - 1, 64);
+            if (a.count_only) {
+                my_pairs += total;
+                continue;
+            }
+            uint64_t base = 0;
+            if (total > 0) {
+                if (lane == 0) base = atomicAdd(a.cursor, (unsigned long long)total);
+                base = wave_bcast_u64(base);
+                if (base + total > a.capacity) {
+                    if (lane == 0) a.deferred[atomicAdd(a.n_deferred, 1u)] = ui;
+                    continue;
+                }
+            }
+            my_pairs += total;
+            wave_lds_sync();
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            const uint32_t qid_l = (uint32_t)lane < nq ? a.ix.ila[qpos0 + lane] : 0u;
+            if ((uint32_t)lane < nq) {
+                a.row_off[qid_l] = a.out_base + base + excl_l;
+                a.row_cnt[qid_l] = cnt_l;
+            }
+            if (total == 0) continue;
+            uint64_t oq[QB];
+            {
+                const uint64_t o_l = base + excl_l;
+#pragma unroll
+                for (int qi = 0; qi < QB; qi++)
+                    oq[qi] = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(o_l >> 32), qi) << 32) |
+                             (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)o_l, qi);
+            }
+            if (keep) {
+                // Chunks with hits (all chunks when the unit has more than the
+                // bitmap holds), kW at a time: their neighbour ids and masks
+                // are loaded together, then written chunk-major, each query's
+                // neighbours in candidate order.
+                constexpr int kW = 4;
+                constexpr uint32_t kNone = ~0u;
+                const uint32_t nwords = (nchunks + 63) / 64;
+                uint32_t wi = 0, nextc = 0;
+                uint64_t wb = hit_bits ? wave_bcast_u64(fl.hitw[0]) : 0ull;
+                auto next_chunk = [&]() __attribute__((always_inline)) -> uint32_t {
+                    if (!hit_bits) return nextc < nchunks ? nextc++ : kNone;
+                    while (wb == 0) {
+                        if (wi + 1 >= nwords) return kNone;
+                        wb = wave_bcast_u64(fl.hitw[++wi]);
+                    }
+                    const uint32_t c = wi * 64 + (uint32_t)__builtin_ctzll(wb);
+                    wb &= wb - 1;
+                    return c;
+                };
+                for (;;) {
+                    uint32_t cl[kW], idr[kW];
+                    uint64_t mkr[kW];
+#pragma unroll
+                    for (int j = 0; j < kW; j++) cl[j] = next_chunk();
+                    if (cl[0] == kNone) break;
+#pragma unroll
+                    for (int j = 0; j < kW; j++) {
+                        idr[j] = 0;
+                        mkr[j] = 0;
+                        const uint32_t c = cl[j];
+                        if (c != kNone) {
+                            if (c < tab_c0 || c >= tab_c0 + (uint32_t)kTab) {
+                                tab_c0 = c;
+                                build_tab(fl, rl, nslots, c);
+                            }
+                            const uint32_t slot = tab_slot(fl, c, tab_c0);
+                            const uint32_t t = c * 64 + lane;
+                            const uint32_t pos = rl.start[slot] + t - rl.off[slot];
+                            idr[j] = gld(a.ix.ila + (t < M ? pos : 0u));
+                            if ((uint32_t)lane < nq) mkr[j] = mask_get(c * nq + lane);
+                        }
+                    }
+#pragma unroll
+                    for (int j = 0; j < kW; j++) {
+                        if (cl[j] == kNone) continue;
+#pragma unroll
+                        for (int qi = 0; qi < QB; qi++) {
+                            // lanes >= nq of the mask hold zero masks
+                            const uint64_t m =
+                                ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(mkr[j] >> 32), qi) << 32) |
+                                (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)mkr[j], qi);
+                            if (m == 0) continue;
+                            if (__builtin_amdgcn_inverse_ballot_w64(m)) gst(a.out + oq[qi] + lanes_below(m), idr[j]);
+                            oq[qi] += (uint64_t)__popcll(m);
+                        }
+                    }
+                }
+            } else {
+                // masks did not fit anywhere (very large units): evaluate each
+                // chunk again with the exact chain, one query at a time.  The
+                // writes stay inside each query's reserved row [oq, oend); a
+                // count that differs from the first pass is reported
+                // (a.mismatch), never written past the row.
+                uint64_t oend[QB];
+                {
+                    const uint64_t e_l = base + excl_l + cnt_l;
+#pragma unroll
+                    for (int qi = 0; qi < QB; qi++)
+                        oend[qi] = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(e_l >> 32), qi) << 32) |
+                                   (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)e_l, qi);
+                }
+                bool bad = false;
+                uint32_t slot2 = 0;
+                for (uint32_t ch = 0; ch < nchunks; ch++) {
+                    const uint32_t t = ch * 64 + lane;
+                    const bool valid = t < M;
+                    const uint32_t pos = valid ? cand_pos_adv(rl, nslots, t, slot2) : 0u;
+                    const uint32_t id = gld(a.ix.ila + pos);
+                    double x[D];
+                    load_cand<D>(a, pos, valid, x);
+                    my_evals += (uint64_t)min(64u, M - ch * 64) * nq;
+#pragma unroll
+                    for (int qi = 0; qi < QB; qi++) {
+                        if ((uint32_t)qi >= nq) continue;
+                        double acc1 = 0.0;
+                        if (a.g.f32dt) {
+                            // DTYPE=float: the reference's FP32 chain
+                            float af = 0.0f;
+#pragma unroll
+                            for (int l = 0; l < D; l++) {
+                                const float dd = (float)x[l] - (float)a.ix.soa[(uint64_t)l * a.g.n + qpos0 + qi];
+                                af = __builtin_fmaf(dd, dd, af);
+                            }
+                            acc1 = (double)af;
+                        } else {
+#pragma unroll
+                            for (int l = 0; l < D; l++) {
+                                const double dd = x[l] - a.ix.soa[(uint64_t)l * a.g.n + qpos0 + qi];
+                                acc1 = __builtin_fma(dd, dd, acc1);
+                            }
+                        }
+                        const uint64_t m = __ballot(valid && acc1 <= T);
+                        if (m == 0) continue;
+                        const uint64_t room = oend[qi] - oq[qi];
+                        const uint64_t pc = (uint64_t)__popcll(m);
+                        bad |= pc > room;
+                        if (__builtin_amdgcn_inverse_ballot_w64(m) && lanes_below(m) < room)
+                            gst(a.out + oq[qi] + lanes_below(m), id);
+                        oq[qi] += pc < room ? pc : room;
+                    }
+                }
+#pragma unroll
+                for (int qi = 0; qi < QB; qi++) bad |= (uint32_t)qi < nq && oq[qi] != oend[qi];
+                if (bad && lane == 0) atomicAdd(a.mismatch, 1ull);
+            }
+        }
+    }
+    if (lane == 0) {
+        if (my_band && a.band) atomicAdd(a.band, (unsigned long long)my_band);
+        if (my_evals) atomicAdd(a.evals, (unsigned long long)my_evals);
+        if (my_pairs) atomicAdd(a.pairs, (unsigned long long)my_pairs);
+    }
+}
+
 // Per-cell candidate count over the stencil (sortByWorkLoadGlobal).
 __global__ __launch_bounds__(256) void k_cell_workload(GridGeom g, DevIndex ix,
                                                        uint32_t *__restrict__ work) {
@@ -1063,9 +1542,9 @@ static hegj_status launch_join_t(const JoinArgs &a, const JoinArgs *ap, uint32_t
     if (a.f32) {
         if constexpr (f32_inst<D, QB>()) {
             if (ap)
-                k_join<D, QB, true, true><<<blocks, 64 * kWavesPerBlock, lds, s>>>(JoinArgs{}, ap);
+                k_join_f32<D, QB, true><<<blocks, 64 * kWavesPerBlock, 0, s>>>(JoinArgs{}, ap);
             else
-                k_join<D, QB, false, true><<<blocks, 64 * kWavesPerBlock, lds, s>>>(a, nullptr);
+                k_join_f32<D, QB, false><<<blocks, 64 * kWavesPerBlock, 0, s>>>(a, nullptr);
         } else {
             return fail(HEGJ_ERR_UNSUPPORTED, "no FP32-filter kernel for this dim / queries_per_unit");
         }
@@ -1105,8 +1584,10 @@ static hegj_status launch_join_qb(const JoinArgs &a, const JoinArgs *ap, uint32_
     }
 }
 
-bool f32_supported(uint32_t dim, uint32_t qb) {
-    return (dim == 2 && (qb == 4 || qb == 8 || qb == 16)) || (dim == 6 && (qb == 4 || qb == 8));
+bool f32_supported(uint32_t dim, uint32_t qb, uint32_t n) {
+    // k_join_f32 addresses the padded FP32 copy with 32-bit byte offsets
+    const bool fits = ((uint64_t)n + kAos32Pad) * dim * sizeof(float) <= 0xffffffffull;
+    return fits && ((dim == 2 && (qb == 4 || qb == 8 || qb == 16)) || (dim == 6 && (qb == 4 || qb == 8)));
 }
 
 hegj_status launch_join(const JoinArgs &a, uint32_t qb, uint32_t blocks, hipStream_t s,
@@ -1186,13 +1667,19 @@ hegj_status zero_words(void *p, size_t bytes, hipStream_t s) {
 namespace hegj {
 
 hegj_status join_occupancy(uint32_t dim, uint32_t qb, int device, uint32_t *blocks,
-                           bool flt) {
+                           JoinKind kind) {
     int per_cu = 0;
     const void *fn = nullptr;
     uint32_t lds = 0;
 #define HEGJ_OCC(D, Q) fn = reinterpret_cast<const void *>(&k_join<D, Q, false>), lds = q_lds_bytes<D, Q>(dim)
     uint32_t d = (dim == 2 || dim == 6 || dim == 18 || dim == 90) ? dim : 0;
-    switch (flt ? qb * 1000 + 999 : qb * 1000 + d) {
+    const uint32_t sel = kind == JoinKind::flt ? qb * 1000 + 999 : kind == JoinKind::f32 ? qb * 1000 + 500 + d : qb * 1000 + d;
+    switch (sel) {
+        case 4502: fn = reinterpret_cast<const void *>(&k_join_f32<2, 4, false>); break;
+        case 8502: fn = reinterpret_cast<const void *>(&k_join_f32<2, 8, false>); break;
+        case 16502: fn = reinterpret_cast<const void *>(&k_join_f32<2, 16, false>); break;
+        case 4506: fn = reinterpret_cast<const void *>(&k_join_f32<6, 4, false>); break;
+        case 8506: fn = reinterpret_cast<const void *>(&k_join_f32<6, 8, false>); break;
         case 4999: fn = reinterpret_cast<const void *>(&k_join<0, 4, false, false, true>), lds = q_lds_bytes<0, 4>(dim); break;
         case 8999: fn = reinterpret_cast<const void *>(&k_join<0, 8, false, false, true>), lds = q_lds_bytes<0, 8>(dim); break;
         case 16999: fn = reinterpret_cast<const void *>(&k_join<0, 16, false, false, true>), lds = q_lds_bytes<0, 16>(dim); break;

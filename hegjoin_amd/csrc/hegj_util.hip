@@ -392,12 +392,14 @@ __global__ __launch_bounds__(256) void k_soa32(const double *__restrict__ soa, u
     if (i < total) out[i] = (float)(soa[i] - centre[i / n]);
 }
 
-// aos32[p * dim + l] = soa32[l * n + p] (the same FP32 values, point-major).
+// aos32[p * dim + l] = soa32[l * n + p] (the same FP32 values, point-major),
+// then kAos32Pad points at +inf (the FP32 join's padding lanes read them).
 __global__ __launch_bounds__(256) void k_aos32(const float *__restrict__ soa32, uint32_t n, uint32_t dim,
                                                float *__restrict__ aos32) {
     const uint32_t p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= n) return;
-    for (uint32_t l = 0; l < dim; l++) aos32[(uint64_t)p * dim + l] = soa32[(uint64_t)l * n + p];
+    if (p >= n + kAos32Pad) return;
+    for (uint32_t l = 0; l < dim; l++)
+        aos32[(uint64_t)p * dim + l] = p < n ? soa32[(uint64_t)l * n + p] : __builtin_inff();
 }
 
 }  // namespace
@@ -434,8 +436,8 @@ hegj_status make_soa32(const GridGeom &g, const DevIndex &ix, float **soa32, flo
     k_soa32<<<ceil_div(total, 256), 256, 0, s>>>(ix.soa, total, n, centre, *soa32);
     HIP_TRY(hipGetLastError());
     if (aos32 && dim <= kAosMaxDim) {
-        HIP_TRY(hipMalloc(aos32, sizeof(float) * (size_t)n * dim));
-        k_aos32<<<ceil_div(n, 256), 256, 0, s>>>(*soa32, n, dim, *aos32);
+        HIP_TRY(hipMalloc(aos32, sizeof(float) * ((size_t)n + kAos32Pad) * dim));
+        k_aos32<<<ceil_div((uint64_t)n + kAos32Pad, 256), 256, 0, s>>>(*soa32, n, dim, *aos32);
         HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipStreamSynchronize(s));
