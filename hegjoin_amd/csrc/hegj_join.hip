@@ -330,21 +330,6 @@ __device__ __forceinline__ void load_cand(const JoinArgs &a, uint32_t pos, bool 
     }
 }
 
-// Same from the centred FP32 copy (point-major).
-template <int D>
-__device__ __forceinline__ void load_cand32(const JoinArgs &a, uint32_t pos, float *x) {
-    static_assert(D <= (int)kAosMaxDim && D % 2 == 0, "FP32 filter: point-major copy, even D");
-    typedef float gf2 __attribute__((ext_vector_type(2)));
-    typedef __attribute__((address_space(1))) const gf2 gf2g;
-    gf2g *src = (gf2g *)(a.aos32 + (uint64_t)pos * D);
-#pragma unroll
-    for (int l = 0; l < D / 2; l++) {
-        const gf2 v = src[l];
-        x[2 * l] = v.x;
-        x[2 * l + 1] = v.y;
-    }
-}
-
 // llvm.amdgcn.writelane (v_writelane_b32): `old` with lane `lane` replaced by
 // the wave-uniform `src`.  Called as the intrinsic, not as inline asm, so
 // the compiler sees the instruction and inserts the wait states it needs
@@ -470,16 +455,14 @@ __device__ __forceinline__ void dist_stream(const JoinArgs &a, uint32_t pos, boo
 // leave room for the rest of the kernel's scalar state (48 of the ~100
 // SGPRs; more spills them to VGPR lanes and every use becomes two
 // v_readlane); otherwise they are staged in LDS, QB * dim doubles per wave.
-// The FP32 filter always keeps its QB * D <= 48 floats in SGPRs.
 template <int D, int QB>
 constexpr bool q_in_sgprs() {
     return D > 0 && D <= 32 && QB * D <= 24;
 }
 
-template <int D, int QB, bool F32 = false>
+template <int D, int QB>
 constexpr uint32_t q_lds_bytes(uint32_t dim) {
-    return (F32 || q_in_sgprs<D, QB>()) ? 0u
-                                         : (uint32_t)(kWavesPerBlockQ * QB * (D > 0 ? D : dim) * sizeof(double));
+    return q_in_sgprs<D, QB>() ? 0u : (uint32_t)(kWavesPerBlockQ * QB * (D > 0 ? D : dim) * sizeof(double));
 }
 
 // Hit masks of one unit: mask[ch * nq + qi], one 64-bit ballot per
@@ -521,25 +504,20 @@ __device__ __forceinline__ bool next_grab(const JoinArgs &a, uint32_t &part, uin
 // latency-bound and needs occupancy; 2 for the other small-D instances up to
 // 8 queries (queries read from LDS next to their use need more vector
 // registers); 1 above.
-template <int D, int QB, bool F32>
+template <int D, int QB>
 constexpr int join_min_waves() {
-    return (D > 0 && D <= 8 && QB <= 8) ? ((F32 || q_in_sgprs<D, QB>()) ? 4 : 2) : 1;
+    return (D > 0 && D <= 8 && QB <= 8) ? (q_in_sgprs<D, QB>() ? 4 : 2) : 1;
 }
 
-// F32: candidates and queries come from the centred FP32 copy; a pair is
-// accepted when acc32 <= tacc32, rejected when acc32 > trej32, and rechecked
-// with the exact FP64 chain in between (the thresholds bound |acc32 - acc|).
-// With DTYPE=float (g.f32dt) the copy is the uncentred float data and
-// tacc32 == trej32 == the float threshold, so acc32 is the reference's own
-// FP32 chain and nothing falls in a band.
+// The exact FP64 chain for every dimensionality (the FP32 filter is
+// k_join_f32 below).
 // FLT: DTYPE=float in the generic instance (D == 0): the FP32 chain over the
 // float coordinates held in the FP64 copies.
-template <int D, int QB, bool IND = false, bool F32 = false, bool FLT = false>
-__global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())) void k_join(
+template <int D, int QB, bool IND = false, bool FLT = false>
+__global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB>())) void k_join(
     JoinArgs av, const JoinArgs *__restrict__ ap) {
     const JoinArgs &a = IND ? *ap : av;
-    static_assert(!F32 || (D > 0 && D <= 8 && QB * D <= 48), "FP32 filter: small D only");
-    static_assert(!FLT || (D == 0 && !F32), "DTYPE=float: generic instance only");
+    static_assert(!FLT || D == 0, "DTYPE=float: generic instance only");
     __shared__ RowList rls[kWavesPerBlock];
     __shared__ HitStore hs_all[kWavesPerBlock];
     extern __shared__ double q_dyn[];  // QB * dim query coordinates per wave
@@ -554,9 +532,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())
     constexpr bool kQS = q_in_sgprs<D, QB>();
     const uint32_t dimq = D > 0 ? (uint32_t)D : a.g.dim;
     double *const qs = q_dyn + (size_t)wid * QB * dimq;
-    double qreg[kQS && !F32 ? QB : 1][kQS && !F32 ? DR : 1];
-    float qregf[F32 ? QB : 1][F32 ? DR : 1];
-    uint64_t my_band = 0;
+    double qreg[kQS ? QB : 1][kQS ? DR : 1];
     uint32_t part = (a.part_by_xcc ? xcc_id() : blockIdx.x) % a.n_parts;
     uint32_t tries = 0, grab = 0, gend = 0;
 
@@ -576,16 +552,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())
                 (int)build_rows<QB>(a, cell, rl, &nslots, &live, qpos0, nq));
             nslots = (uint32_t)__builtin_amdgcn_readfirstlane((int)nslots);
             // the unit's query coordinates, once: lane i loads element i
-            if constexpr (F32) {
-                constexpr int NQD = QB * DR;
-                const int qi = lane / DR, l = lane % DR;
-                float v = 0.0f;
-                if (lane < NQD && (uint32_t)qi < nq) v = a.soa32[(uint64_t)l * a.g.n + qpos0 + qi];
-#pragma unroll
-                for (int j = 0; j < NQD; j++)
-                    qregf[j / DR][j % DR] =
-                        __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(v), j));
-            } else if (kQS) {
+            if constexpr (kQS) {
                 constexpr int NQD = kQS ? QB * DR : 1;
 #pragma unroll
                 for (int i0 = 0; i0 < NQD; i0 += 64) {
@@ -599,7 +566,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())
                         const uint64_t u = (uint64_t)__double_as_longlong(v);
                         const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, j - i0);
                         const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), j - i0);
-                        qreg[kQS && !F32 ? j / DR : 0][kQS && !F32 ? j % DR : 0] =
+                        qreg[kQS ? j / DR : 0][kQS ? j % DR : 0] =
                             __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
                     }
                 }
@@ -630,23 +597,14 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())
             };
             uint32_t cnt_l = 0;  // lane qi: hits of query qi so far
             uint64_t mine = 0;   // lane qi: query qi's mask of the current chunk
-            uint32_t nband = 0;  // FP32 filter: (query, chunk) pairs with lanes in the band
 
             uint32_t slot = 0;
             // Candidate coordinates are prefetched kDepth chunks ahead in a
             // register ring (the loop is unrolled by kDepth so ring indices
             // are static): several chunks of loads stay in flight per wave.
             constexpr int kDepth = kRegs ? (D <= 8 ? 4 : 2) : 1;
-            double ring[F32 ? 1 : kDepth][DR];
-            float ringf[F32 ? kDepth : 1][DR];
-            if constexpr (F32) {
-#pragma unroll
-                for (int p = 0; p < kDepth; p++) {
-                    const uint32_t t = p * 64 + lane;
-                    const bool v = t < M;
-                    load_cand32<DR>(a, v ? cand_pos_adv(rl, nslots, t, slot) : 0u, ringf[p]);
-                }
-            } else if (kRegs) {
+            double ring[kDepth][DR];
+            if constexpr (kRegs) {
 #pragma unroll
                 for (int p = 0; p < kDepth; p++) {
                     const uint32_t t = p * 64 + lane;
@@ -678,44 +636,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())
                         // single compares feed the ballots; validity is applied
                         // to the scalar masks
                         const uint64_t vmask = __builtin_amdgcn_ballot_w64(valid);
-                        if constexpr (F32) {
-                            float accf[QL];
-#pragma unroll
-                            for (int qi = 0; qi < QL; qi++) {
-                                accf[qi] = 0.0f;
-#pragma unroll
-                                for (int l = 0; l < DR; l++) {
-                                    const float dd = ringf[p][l] - qregf[qi][l];
-                                    accf[qi] = __builtin_fmaf(dd, dd, accf[qi]);
-                                }
-                            }
-                            {
-                                // refill the ring slot only after the distances
-                                // that read it: the new values then take the same
-                                // registers and the loop carries the ring without
-                                // copies (a copy of a value still loading forces a
-                                // wait for every load in flight)
-                                if constexpr (QL > 1) {
-#pragma unroll
-                                    for (int qi = 0; qi < QL; qi++) __asm__ volatile("" ::"v"(accf[qi]) : "memory");
-                                }
-                                const uint32_t tn = (ch + kDepth) * 64 + lane;
-                                load_cand32<DR>(a, tn < M ? cand_pos_adv(rl, nslots, tn, slot) : 0u, ringf[p]);
-                            }
-#pragma unroll
-                            for (int qi = 0; qi < QL; qi++) {
-                                const uint64_t vq = (uint32_t)qi < nq ? vmask : 0ull;
-                                const uint64_t m = __builtin_amdgcn_ballot_w64(accf[qi] <= a.tacc32) & vq;
-                                const uint64_t mb = vq & ~m & ~__builtin_amdgcn_ballot_w64(accf[qi] > a.trej32);
-                                if (mb) {
-                                    // decided after the loop by the exact FP64 chain
-                                    if (lane == 0 && nband < (uint32_t)kBandList) rl.band[nband] = (ch << 4) | (uint32_t)qi;
-                                    nband++;
-                                    my_band += (uint64_t)__popcll(mb);
-                                }
-                                mine = set_lane_u64(mine, m, qi);
-                            }
-                        } else {
+                        {
                             double acc[QL];
                             if constexpr (kRegs) {
                                 // use ring[p], then refill it with chunk ch + kDepth
@@ -758,54 +679,6 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())
                     chunk_loop(std::integral_constant<int, QB>{});
             } else {
                 chunk_loop(std::integral_constant<int, QB>{});
-            }
-            if constexpr (F32) {
-                // The band: the reference's exact FP64 chain decides the lanes
-                // whose FP32 sum lies between the two thresholds.  Listed
-                // (query, chunk) pairs, or all of them when the list overflowed.
-                if (nband) {
-                    wave_lds_sync();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-                    const bool all = nband > (uint32_t)kBandList;
-                    const uint32_t nitems = all ? nq * nchunks : nband;
-                    const uint64_t n = a.g.n;
-                    for (uint32_t e = 0; e < nitems; e++) {
-                        uint32_t qi, ch;
-                        if (all) {
-                            ch = e / nq;
-                            qi = e - ch * nq;
-                        } else {
-                            const uint32_t v = rl.band[e];
-                            qi = v & 15u;
-                            ch = v >> 4;
-                        }
-                        const uint32_t t = ch * 64 + lane;
-                        const bool valid = t < M;
-                        const uint32_t pos = valid ? cand_pos(rl, nslots, t) : 0u;
-                        float af = 0.0f;
-#pragma unroll
-                        for (int l = 0; l < DR; l++) {
-                            const float dd = a.soa32[(uint64_t)l * n + pos] - a.soa32[(uint64_t)l * n + qpos0 + qi];
-                            af = __builtin_fmaf(dd, dd, af);
-                        }
-                        const bool band = valid && !(af <= a.tacc32) && !(af > a.trej32);
-                        bool hit = false;
-                        if (band) {
-                            double e64 = 0.0;
-#pragma unroll
-                            for (int l = 0; l < DR; l++) {
-                                const double dd = a.ix.soa[(uint64_t)l * n + pos] - a.ix.soa[(uint64_t)l * n + qpos0 + qi];
-                                e64 = __builtin_fma(dd, dd, e64);
-                            }
-                            hit = e64 <= T;
-                        }
-                        const uint64_t madd = __ballot(hit);
-                        if (madd) {
-                            if (keep && lane == 0) mask_put(ch * nq + qi, mask_get(ch * nq + qi) | madd);
-                            cnt_l += (uint32_t)lane == qi ? (uint32_t)__popcll(madd) : 0u;
-                        }
-                    }
-                }
             }
             my_evals += (uint64_t)M * nq;
             // lane qi: first slot of query qi within the unit's segment
@@ -918,24 +791,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())
                     for (int qi = 0; qi < QB; qi++) {
                         if ((uint32_t)qi >= nq) continue;
                         double acc1 = 0.0;
-                        if constexpr (F32) {
-                            if (a.g.f32dt) {
-                                // DTYPE=float: the reference's FP32 chain
-                                float af = 0.0f;
-#pragma unroll
-                                for (int l = 0; l < DR; l++) {
-                                    const float dd = (float)x[l] - (float)a.ix.soa[(uint64_t)l * a.g.n + qpos0 + qi];
-                                    af = __builtin_fmaf(dd, dd, af);
-                                }
-                                acc1 = (double)af;
-                            } else {
-#pragma unroll
-                                for (int l = 0; l < DR; l++) {
-                                    const double dd = x[l] - a.ix.soa[(uint64_t)l * a.g.n + qpos0 + qi];
-                                    acc1 = __builtin_fma(dd, dd, acc1);
-                                }
-                            }
-                        } else if constexpr (kRegs) {
+                        if constexpr (kRegs) {
 #pragma unroll
                             for (int l = 0; l < DR; l++) {
                                 const double q = kQS ? qreg[kQS ? qi : 0][kQS ? l : 0] : qs[qi * DR + l];
@@ -963,7 +819,6 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB, F32>())
     }
     // one atomic per wave for the counters
     if (lane == 0) {
-        if (F32 && my_band && a.band) atomicAdd(a.band, (unsigned long long)my_band);
         if (my_evals) atomicAdd(a.evals, (unsigned long long)my_evals);
         if (my_pairs) atomicAdd(a.pairs, (unsigned long long)my_pairs);
     }
@@ -1536,7 +1391,7 @@ constexpr bool f32_inst() {
 template <int D, int QB>
 static hegj_status launch_join_t(const JoinArgs &a, const JoinArgs *ap, uint32_t blocks,
                                  hipStream_t s) {
-    const uint32_t lds = a.f32 ? q_lds_bytes<D, QB, true>(a.g.dim) : q_lds_bytes<D, QB>(a.g.dim);
+    const uint32_t lds = a.f32 ? 0u : q_lds_bytes<D, QB>(a.g.dim);
     if (lds > kMaxQueryLds)
         return fail(HEGJ_ERR_UNSUPPORTED, "queries_per_unit * dim too large for LDS; lower queries_per_unit");
     if (a.f32) {
@@ -1566,7 +1421,7 @@ static hegj_status launch_join_flt(const JoinArgs &a, const JoinArgs *ap, uint32
     const uint32_t lds = q_lds_bytes<0, QB>(a.g.dim);
     if (lds > kMaxQueryLds)
         return fail(HEGJ_ERR_UNSUPPORTED, "queries_per_unit * dim too large for LDS; lower queries_per_unit");
-    k_join<0, QB, false, false, true><<<blocks, 64 * kWavesPerBlock, lds, s>>>(a, nullptr);
+    k_join<0, QB, false, true><<<blocks, 64 * kWavesPerBlock, lds, s>>>(a, nullptr);
     HIP_TRY(hipGetLastError());
     return HEGJ_OK;
 }
@@ -1680,9 +1535,9 @@ hegj_status join_occupancy(uint32_t dim, uint32_t qb, int device, uint32_t *bloc
         case 16502: fn = reinterpret_cast<const void *>(&k_join_f32<2, 16, false>); break;
         case 4506: fn = reinterpret_cast<const void *>(&k_join_f32<6, 4, false>); break;
         case 8506: fn = reinterpret_cast<const void *>(&k_join_f32<6, 8, false>); break;
-        case 4999: fn = reinterpret_cast<const void *>(&k_join<0, 4, false, false, true>), lds = q_lds_bytes<0, 4>(dim); break;
-        case 8999: fn = reinterpret_cast<const void *>(&k_join<0, 8, false, false, true>), lds = q_lds_bytes<0, 8>(dim); break;
-        case 16999: fn = reinterpret_cast<const void *>(&k_join<0, 16, false, false, true>), lds = q_lds_bytes<0, 16>(dim); break;
+        case 4999: fn = reinterpret_cast<const void *>(&k_join<0, 4, false, true>), lds = q_lds_bytes<0, 4>(dim); break;
+        case 8999: fn = reinterpret_cast<const void *>(&k_join<0, 8, false, true>), lds = q_lds_bytes<0, 8>(dim); break;
+        case 16999: fn = reinterpret_cast<const void *>(&k_join<0, 16, false, true>), lds = q_lds_bytes<0, 16>(dim); break;
         case 4002: HEGJ_OCC(2, 4); break;
         case 4006: HEGJ_OCC(6, 4); break;
         case 4018: HEGJ_OCC(18, 4); break;

@@ -277,6 +277,9 @@ void hegj_server::worker(uint32_t slot) {
         }
         hegj_status st = init_err.empty() ? run(r, s, arena, ctr) : fail(HEGJ_ERR_HIP, init_err);
         std::string msg = st == HEGJ_OK ? std::string() : std::string(hegj_last_error());
+        // a failed request's kernels may still be writing into its pool range:
+        // drain the slot's stream before the range can go to another request
+        if (st != HEGJ_OK && s) (void)hipStreamSynchronize(s);
         {
             std::lock_guard<std::mutex> lk(mu);
             r->t_done = Clock::now();
