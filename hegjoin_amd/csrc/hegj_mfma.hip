@@ -38,7 +38,11 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kGbWaves = 4;          // waves per workgroup
+// Waves per workgroup: the double-buffered candidate tile (64 rows of K
+// bf16) takes 2 * 64 * (K + 8) * 2 bytes of LDS, so above K = 128 one
+// workgroup fills a CU's LDS and its waves are all the CU runs; more waves
+// then both fill the SIMDs and serve each streamed candidate to more queries.
+constexpr int gb_waves(int kb) { return kb >= 6 ? 8 : 4; }
 constexpr int kGbChunk = 64;         // candidates per LDS chunk (4 col tiles)
 // 16-query row tiles per wave: 4 while the A fragments are small, 2 above
 constexpr int gb_rt(int kb) { return kb <= 4 ? 4 : 2; }
@@ -254,7 +258,8 @@ __device__ __forceinline__ void close_appender(Appender &ap, uint64_t *list) {
 }
 
 template <int KB>
-__global__ __launch_bounds__(64 * kGbWaves) void k_gb_join(GbArgs a) {
+__global__ __launch_bounds__(64 * gb_waves(KB)) void k_gb_join(GbArgs a) {
+    constexpr int kGbWaves = gb_waves(KB);
     constexpr int KP = KB * 32;
     constexpr int ROW = KP + 8;  // padded LDS row (bf16) against bank conflicts
     constexpr int RT = gb_rt(KB);
@@ -359,8 +364,8 @@ __global__ __launch_bounds__(64 * kGbWaves) void k_gb_join(GbArgs a) {
         constexpr int kU4 = KP * 2 / 16;                                   // 16B units per row
         constexpr int kPieceU4 = (kU4 + kGbWaves - 1) / kGbWaves;          // per thread
         uint32_t slot = 0;
-        uint4 stage[kPieceU4];
-        auto fetch = [&](uint32_t ch) {
+        uint4 stA[kPieceU4], stB[kPieceU4];
+        auto fetch = [&](uint32_t ch, uint4(&stage)[kPieceU4]) {
             const uint32_t t = ch * kGbChunk + cand;
             const bool v = t < M;
             const uint32_t pos = v ? gb_cand_pos(rl, nslots, t, slot) : 0u;
@@ -386,7 +391,7 @@ __global__ __launch_bounds__(64 * kGbWaves) void k_gb_join(GbArgs a) {
             }
             return pos;
         };
-        auto store = [&](uint32_t buf, uint32_t pos) {
+        auto store = [&](uint32_t buf, const uint4(&stage)[kPieceU4], uint32_t pos) {
             uint4 *dst = reinterpret_cast<uint4 *>(&cb[buf][cand * ROW]);
 #pragma unroll
             for (int i = 0; i < kPieceU4; i++) {
@@ -396,13 +401,13 @@ __global__ __launch_bounds__(64 * kGbWaves) void k_gb_join(GbArgs a) {
             if (piece == 0) cpos[buf][cand] = pos;
         };
 
-        if (nchunks > 0) store(0, fetch(0));
-        __syncthreads();
-        for (uint32_t ch = 0; ch < nchunks; ch++) {
+        // Two chunks of loads in flight: chunk ch + 2 is loaded into one
+        // register stage while chunk ch is computed, and stored into LDS at
+        // the end of chunk ch + 1.
+        auto step = [&](uint32_t ch, const uint4(&s_next)[kPieceU4], uint32_t pos_next, uint4(&s_new)[kPieceU4],
+                        uint32_t &pos_new) __attribute__((always_inline)) {
             const uint32_t buf = ch & 1;
-            uint32_t npos = 0;
-            const bool more = ch + 1 < nchunks;
-            if (more) npos = fetch(ch + 1);  // global loads in flight during the MFMAs
+            if (ch + 2 < nchunks) pos_new = fetch(ch + 2, s_new);
             f32x4 acc[RT][4];
 #pragma unroll
             for (int rt = 0; rt < RT; rt++)
@@ -476,8 +481,17 @@ __global__ __launch_bounds__(64 * kGbWaves) void k_gb_join(GbArgs a) {
                         wave_lds_sync();
                     }
             }
-            if (more) store(buf ^ 1, npos);
+            if (ch + 1 < nchunks) store(buf ^ 1, s_next, pos_next);
             __syncthreads();
+        };
+        uint32_t posA = 0, posB = 0;
+        if (nchunks > 0) store(0, stA, fetch(0, stA));
+        if (nchunks > 1) posB = fetch(1, stB);
+        __syncthreads();
+        for (uint32_t ch = 0; ch < nchunks; ch += 2) {
+            step(ch, stB, posB, stA, posA);
+            if (ch + 1 >= nchunks) break;
+            step(ch + 1, stA, posA, stB, posB);
         }
         if (wid == 0 && lane == 0) my_evals += (uint64_t)M * nq;
     }
@@ -577,7 +591,7 @@ __global__ __launch_bounds__(256) void k_u32_u64(const uint32_t *__restrict__ in
 
 template <int KB>
 hegj_status launch_gb(const GbArgs &a, uint32_t blocks, hipStream_t s) {
-    k_gb_join<KB><<<blocks, 64 * kGbWaves, 0, s>>>(a);
+    k_gb_join<KB><<<blocks, 64 * gb_waves(KB), 0, s>>>(a);
     HIP_TRY(hipGetLastError());
     return HEGJ_OK;
 }
@@ -586,7 +600,7 @@ template <int KB>
 hegj_status occ_gb(int device, uint32_t *blocks) {
     int per_cu = 0;
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void *>(&k_gb_join<KB>), 64 * kGbWaves, 0));
+        &per_cu, reinterpret_cast<const void *>(&k_gb_join<KB>), 64 * gb_waves(KB), 0));
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, device));
     *blocks = (uint32_t)std::max(per_cu, 1) * (uint32_t)prop.multiProcessorCount;
@@ -598,7 +612,7 @@ hegj_status occ_gb(int device, uint32_t *blocks) {
 // K blocks (of 32) instantiated for the MFMA refine.
 static const int kGbKBs[] = {2, 3, 4, 6, 8, 9, 12};
 
-uint32_t gb_unit_queries(uint32_t kb) { return kGbWaves * gb_rt((int)kb) * 16; }
+uint32_t gb_unit_queries(uint32_t kb) { return gb_waves((int)kb) * gb_rt((int)kb) * 16; }
 
 uint32_t gb_kblocks(uint32_t dim) {
     uint32_t need = (3 * (dim + 2) + 31) / 32;
@@ -649,7 +663,7 @@ hegj_status gb_prepare(const GridGeom &g, const DevIndex &ix, uint32_t kp, uint1
 
 hegj_status gb_join(const GridGeom &g, const DevIndex &ix, const uint2 *units, uint32_t n_units,
                     const uint16_t *Av, const uint16_t *Bv, const float *w, uint32_t kp,
-                    int device, hipStream_t s, GbOutput *out) {
+                    int device, hipStream_t s, GbOutput *out, uint64_t pair_cap_first) {
     const uint32_t kb = kp / 32;
     uint32_t max_blocks = 0;
     switch (kb) {
@@ -663,7 +677,8 @@ hegj_status gb_join(const GridGeom &g, const DevIndex &ix, const uint2 *units, u
         default: return fail(HEGJ_ERR_UNSUPPORTED, "no MFMA kernel for this dimensionality");
     }
     const uint32_t n = g.n;
-    uint64_t pair_cap = std::max<uint64_t>((uint64_t)n * 48, 1u << 22);
+    uint64_t pair_cap = pair_cap_first ? std::max<uint64_t>(pair_cap_first, 4096)
+                                       : std::max<uint64_t>((uint64_t)n * 48, 1u << 22);
     uint64_t band_cap = std::max<uint64_t>((uint64_t)n * 8, 1u << 22);
     struct Ctr {
         uint32_t queue_head, overflow;

@@ -202,7 +202,10 @@ typedef enum {
 } hegj_refine;
 
 typedef struct {
-    uint64_t result_capacity; /* pairs per device result batch; 0 = estimate */
+    uint64_t result_capacity; /* pairs per device result batch; 0 = estimate.  MFMA
+                                 refine: the size of its first pair list (the join's
+                                 pairs are held in one list, 8 bytes each; a list
+                                 that overflows is rebuilt once at 1.25x the count) */
     uint32_t unit_order;      /* hegj_unit_order (default: Morton) */
     uint32_t queries_per_unit;/* FP64 kernel: 4, 8 or 16; 0 = tuned default */
     uint32_t refine;          /* hegj_refine */

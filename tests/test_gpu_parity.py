@@ -197,6 +197,17 @@ def test_mfma_refine_vs_oracle(H, oracle, dim, k, n, eps, seed):
     assert info["pair_hash"] == oracle.csr_hash(eoff, enb)
 
 
+def test_mfma_pair_list_rebuilt_after_overflow(H, oracle):
+    """result_capacity sizes the MFMA refine's first pair list; a list far
+    too small for the join overflows and the refine runs again with a list
+    sized from the count it reached: same pair set as the oracle."""
+    pts = oracle.generate_expo(4000, 18, seed=22)
+    off, nb, info = gpu_join(H, pts, 0.15, 6, refine="mfma", result_capacity=4096)
+    eoff, enb = oracle.join(pts, 6, 0.15)
+    assert info["refine_used"] == "mfma" and info["total_neighbors"] > 4096
+    assert np.array_equal(off, eoff) and np.array_equal(nb, enb)
+
+
 def planted_boundary(dim, n_base, eps, seed):
     """Random points plus, for each, partners at distance eps (1 + r) with
     |r| <= 1e-9 along random directions: pairs whose FP64 distance sits on
