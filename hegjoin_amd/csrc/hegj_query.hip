@@ -16,12 +16,15 @@
 //                   on each side (coordinates -1 .. nCells), or `dead` when
 //                   the query lies further out (no point can be in range);
 //   radix sort      (key, query id), dead keys last;
-//   k_qruns         one unit per run of equal keys (queries sharing a cell
-//                   share the candidate rows);
+//   k_qruns         one unit per run of equal keys (up to 16 queries of one
+//                   cell); each group of 4 of them builds its stencil rows,
+//                   pruned to the cells within eps of one of its queries;
 //   k_qjoin<count>  per-query neighbour counts;
 //   scan            CSR offsets, exact result size (admission happens here);
 //   k_qjoin<fill>   each query's neighbours written at its own offset in
 //                   candidate (cell) order -- no atomics, deterministic.
+#include <type_traits>
+
 #include "hegj_internal.h"
 
 namespace hegj {
@@ -34,6 +37,7 @@ constexpr int kQB = 4;  // queries evaluated together against one candidate chun
 struct QRowList {
     uint32_t start[kMaxRows];
     uint32_t off[kMaxRows + 1];
+    float2 gap[kQB * kMaxK];  // per (query, indexed dim): squared gaps to the -1 / +1 cell
 };
 
 __global__ __launch_bounds__(256) void k_qcell_keys(const double *__restrict__ q, uint32_t nq,
@@ -97,12 +101,23 @@ __global__ __launch_bounds__(256) void k_sub_fill(const uint32_t *__restrict__ s
     for (uint32_t i = b; i < e; i += kUnitQueries) sub[o++] = i;
 }
 
-// Stencil rows of the unit whose widened key is `key` into the wave's LDS
-// list; returns the candidate count.
-__device__ uint32_t qbuild_rows(const QJoinArgs &a, uint64_t key, QRowList &rl, uint32_t *nslots) {
+// Stencil rows of a group of queries (all in the cell whose widened key is
+// `key`) into the wave's LDS list; returns the candidate count.
+//
+// Cell-box pruning, as in the self-join: a stencil cell whose box lies
+// farther than eps from every query of the group holds no neighbour and is
+// left out.  With f_j the query's position inside its cell along indexed dim
+// j (in cell sides), the gap to the -1 cell is f_j, to the +1 cell 1 - f_j;
+// a cell is kept when the smallest sum of squared gaps over the group is at
+// most T / side^2.  Gaps are shrunk by 1e-6 + 2e-15 (|c_j| + 2) (the FP64
+// rounding of the cell assignments) and the bound raised by 1e-5 relative
+// (FP32 sums), so the kept set can only be larger than the exact one.
+__device__ uint32_t qbuild_rows(const QJoinArgs &a, uint64_t key, const uint32_t *qid, uint32_t nqg,
+                                QRowList &rl, uint32_t *nslots) {
     const GridGeom &g = a.qg.g;
     const int lane = threadIdx.x & 63;
     uint32_t c[kMaxK], lo[kMaxK], hi[kMaxK];
+    int64_t cs[kMaxK];
     bool ok = true;
     for (uint32_t j = 0; j < g.k; j++) {
         const uint64_t w = (uint64_t)g.nCells[j] + 2;
@@ -113,6 +128,7 @@ __device__ uint32_t qbuild_rows(const QJoinArgs &a, uint64_t key, QRowList &rl, 
         const int64_t h = cj + 1 > (int64_t)g.nCells[j] - 1 ? (int64_t)g.nCells[j] - 1 : cj + 1;
         ok &= l <= h;
         c[j] = (uint32_t)cj;  // -1 wraps; stencil_row adds the offset modulo 2^32
+        cs[j] = cj;
         lo[j] = (uint32_t)l;
         hi[j] = (uint32_t)(h < 0 ? 0 : h);
     }
@@ -120,17 +136,75 @@ __device__ uint32_t qbuild_rows(const QJoinArgs &a, uint64_t key, QRowList &rl, 
         *nslots = 0;
         return 0;
     }
-    const uint32_t nrows = pow3(g.k - 1);
+    const uint32_t k = g.k, dim = g.dim;
+    const double inv_side = 1.0 / g.eps;
+    wave_lds_sync();  // the previous group's reads of the list are done
+    for (uint32_t e = (uint32_t)lane; e < nqg * k; e += 64) {
+        const uint32_t qi = e / k, j = e - qi * k;
+        uint32_t id = 0;
+        int64_t cj = 0;
+#pragma unroll
+        for (int t = 0; t < kQB; t++) id = (uint32_t)t == qi ? qid[t] : id;
+#pragma unroll
+        for (int jj = 0; jj < kMaxK; jj++) cj = (uint32_t)jj == j ? cs[jj] : cj;
+        const double x = a.queries[(uint64_t)id * dim + j];
+        double f = (x - g.minArr[j]) * inv_side - (double)cj;
+        f = f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
+        const double mg = 1e-6 + 2e-15 * ((double)(cj < 0 ? -cj : cj) + 2.0);
+        const double gm = f - mg > 0.0 ? f - mg : 0.0, gp = 1.0 - f - mg > 0.0 ? 1.0 - f - mg : 0.0;
+        rl.gap[qi * kMaxK + j] = make_float2((float)(gm * gm), (float)(gp * gp));
+    }
+    wave_lds_sync();
+    const float thr = (float)(a.thresh * inv_side * inv_side) * (1.0f + 1e-5f);
+    const uint32_t nrows = pow3(k - 1);
     uint32_t slots = 0, total = 0;
     for (uint32_t r0 = 0; r0 < nrows; r0 += 64) {
         const uint32_t r = r0 + lane;
         uint32_t pstart = 0, plen = 0;
         if (r < nrows) {
-            uint32_t cf, cl;
-            stencil_row(c, lo, hi, r, g, a.ix, &cf, &cl);
-            if (cl > cf) {
-                pstart = a.ix.cell_start[cf];
-                plen = a.ix.cell_start[cl] - pstart;
+            // offset digits of row r along dims 1..k-1 (0: -1, 1: 0, 2: +1)
+            uint32_t o[kMaxK];
+            uint32_t rr = r;
+#pragma unroll
+            for (int j = 1; j < kMaxK; j++) {
+                o[j] = rr % 3;
+                rr /= 3;
+            }
+            float mmid = 3.4e38f, mleft = 3.4e38f, mright = 3.4e38f;
+            for (uint32_t qi = 0; qi < nqg; qi++) {
+                float rd = 0.0f;
+#pragma unroll
+                for (int j = 1; j < kMaxK; j++) {
+                    if ((uint32_t)j >= k) break;
+                    const float2 v = rl.gap[qi * kMaxK + j];
+                    rd += o[j] == 0 ? v.x : (o[j] == 2 ? v.y : 0.0f);
+                }
+                const float2 v0 = rl.gap[qi * kMaxK];
+                mmid = fminf(mmid, rd);
+                mleft = fminf(mleft, rd + v0.x);
+                mright = fminf(mright, rd + v0.y);
+            }
+            if (mmid <= thr) {
+                // dim-0 window, without the -1 / +1 cell when no query reaches it
+                int64_t wlo = lo[0], whi = hi[0];
+                if (!(mleft <= thr) && cs[0] > wlo) wlo = cs[0];
+                if (!(mright <= thr) && cs[0] < whi) whi = cs[0];
+                if (wlo <= whi) {
+                    uint32_t lo2[kMaxK], hi2[kMaxK];
+#pragma unroll
+                    for (int j = 0; j < kMaxK; j++) {
+                        lo2[j] = lo[j];
+                        hi2[j] = hi[j];
+                    }
+                    lo2[0] = (uint32_t)wlo;
+                    hi2[0] = (uint32_t)whi;
+                    uint32_t cf, cl;
+                    stencil_row(c, lo2, hi2, r, g, a.ix, &cf, &cl);
+                    if (cl > cf) {
+                        pstart = a.ix.cell_start[cf];
+                        plen = a.ix.cell_start[cl] - pstart;
+                    }
+                }
             }
         }
         const uint64_t nz = __ballot(plen > 0);
@@ -150,6 +224,7 @@ __device__ uint32_t qbuild_rows(const QJoinArgs &a, uint64_t key, QRowList &rl, 
     }
     if (lane == 0) rl.off[slots] = total;
     *nslots = slots;
+    wave_lds_sync();
     return total;
 }
 
@@ -175,9 +250,7 @@ __global__ __launch_bounds__(64 * kQWaves) void k_qjoin(QJoinArgs a) {
         u = __shfl(u, 0, 64);
         if (u >= a.n_units) break;
         const uint32_t s0 = a.starts[u], s1 = a.starts[u + 1];
-        uint32_t nslots;
-        const uint32_t M = qbuild_rows(a, a.keys[s0], rl, &nslots);
-        wave_lds_sync();
+        const uint64_t ukey = a.keys[s0];
         for (uint32_t g0 = s0; g0 < s1; g0 += kQB) {
             const uint32_t nqg = min((uint32_t)kQB, s1 - g0);
             uint32_t qid[kQB];
@@ -189,6 +262,8 @@ __global__ __launch_bounds__(64 * kQWaves) void k_qjoin(QJoinArgs a) {
                 base[qi] = FILL ? a.offsets[qid[qi]] : 0;
                 cnt[qi] = 0;
             }
+            uint32_t nslots;
+            const uint32_t M = qbuild_rows(a, ukey, qid, nqg, rl, &nslots);
             // the group's query coordinates, once (lane i loads element i;
             // scalar registers), so no load in the candidate loop waits on
             // anything but the candidates
@@ -226,103 +301,207 @@ __global__ __launch_bounds__(64 * kQWaves) void k_qjoin(QJoinArgs a) {
                 }
             }
             uint32_t slot = 0;
-            for (uint32_t t0 = 0; t0 < M; t0 += 64) {
-                const uint32_t t = t0 + lane;
-                const bool valid = t < M;
-                uint32_t pos = 0;
-                if (valid) {
-                    while (slot + 1 < nslots && rl.off[slot + 1] <= t) slot++;
-                    pos = rl.start[slot] + (t - rl.off[slot]);
-                }
-                // one compare per ballot, validity applied to the scalar
-                // masks; the hit lanes of a fill store under the mask itself
-                const uint64_t vmask = __builtin_amdgcn_ballot_w64(valid);
-                uint32_t pid = 0;
-                if (FILL) pid = a.ix.ila[valid ? pos : 0u];
-                double acc[kQB];
+            if constexpr (D > 0) {
+                // Candidate chunks kDepth ahead in a register ring (coordinates,
+                // positions and, when filling, point ids), so a wave keeps
+                // several chunks of loads in flight instead of waiting on each.
+                // A slot's position and id are copied out before the slot is
+                // refilled; its coordinates are read by the distances first.
+                constexpr int kDepth = 4;
+                typedef typename std::conditional<F32, float, double>::type CT;
+                CT ring[kDepth][DQ];
+                uint32_t posr[kDepth], pidr[kDepth];
+                auto load = [&](uint32_t t0, int p) __attribute__((always_inline)) {
+                    const uint32_t t = t0 + lane;
+                    uint32_t pos = 0;
+                    if (t < M) {
+                        while (slot + 1 < nslots && rl.off[slot + 1] <= t) slot++;
+                        pos = rl.start[slot] + (t - rl.off[slot]);
+                    }
+                    posr[p] = pos;
+                    if constexpr (F32) {
+                        if constexpr (DQ % 2 == 0) {
+                            typedef float f2 __attribute__((ext_vector_type(2)));
+                            const f2 *src = reinterpret_cast<const f2 *>(a.aos32 + (uint64_t)pos * DQ);
 #pragma unroll
-                for (int qi = 0; qi < kQB; qi++) acc[qi] = 0.0;
-                if constexpr (F32) {
-                    float x[DQ];
-                    if constexpr (DQ % 2 == 0) {
-                        typedef float f2 __attribute__((ext_vector_type(2)));
-                        const f2 *src = reinterpret_cast<const f2 *>(a.aos32 + (uint64_t)pos * DQ);
+                            for (int l = 0; l < DQ / 2; l++) {
+                                const f2 v = src[l];
+                                ring[p][2 * l] = v.x;
+                                ring[p][2 * l + 1] = v.y;
+                            }
+                        } else {
 #pragma unroll
-                        for (int l = 0; l < DQ / 2; l++) {
-                            const f2 v = src[l];
-                            x[2 * l] = v.x;
-                            x[2 * l + 1] = v.y;
+                            for (int l = 0; l < DQ; l++) ring[p][l] = a.soa32[(uint64_t)l * n + pos];
                         }
+                    } else if constexpr (D <= (int)kAosMaxDim) {
+                        const double *src = a.ix.aos + (uint64_t)pos * DQ;
+#pragma unroll
+                        for (int l = 0; l < DQ; l++) ring[p][l] = src[l];
                     } else {
 #pragma unroll
-                        for (int l = 0; l < DQ; l++) x[l] = a.soa32[(uint64_t)l * n + pos];
+                        for (int l = 0; l < DQ; l++) ring[p][l] = soa[(uint64_t)l * n + pos];
                     }
+                    if (FILL) pidr[p] = a.ix.ila[pos];
+                };
 #pragma unroll
-                    for (int qi = 0; qi < kQB; qi++) {
-                        float accf = 0.0f;
+                for (int p = 0; p < kDepth; p++) load((uint32_t)p * 64, p);
+                const uint32_t nch = (M + 63) / 64;
+                for (uint32_t c0 = 0; c0 < nch; c0 += kDepth) {
 #pragma unroll
-                        for (int l = 0; l < DQ; l++) {
-                            const float d = x[l] - qregf[qi][l];
-                            accf = __builtin_fmaf(d, d, accf);
-                        }
-                        const uint64_t vq = (uint32_t)qi < nqg ? vmask : 0ull;
-                        uint64_t m = __builtin_amdgcn_ballot_w64(accf <= a.tacc32) & vq;
-                        const uint64_t mb = vq & ~m & ~__builtin_amdgcn_ballot_w64(accf > a.trej32);
-                        if (mb) {
-                            // the band: the exact FP64 chain decides
-                            bool hit = false;
-                            if (__builtin_amdgcn_inverse_ballot_w64(mb)) {
-                                double e = 0.0;
+                    for (int p = 0; p < kDepth; p++) {
+                        const uint32_t t0 = (c0 + p) * 64;
+                        const bool valid = t0 + lane < M;
+                        const uint64_t vmask = __builtin_amdgcn_ballot_w64(valid);
+                        const uint32_t pos = posr[p];
+                        const uint32_t pid = FILL ? pidr[p] : 0u;
+                        CT accv[kQB];
 #pragma unroll
-                                for (int l = 0; l < DQ; l++) {
-                                    const double d = soa[(uint64_t)l * n + pos] - a.queries[(uint64_t)qid[qi] * dim + l];
-                                    e = __builtin_fma(d, d, e);
+                        for (int qi = 0; qi < kQB; qi++) {
+                            accv[qi] = (CT)0;
+#pragma unroll
+                            for (int l = 0; l < DQ; l++) {
+                                if constexpr (F32) {
+                                    const float d = ring[p][l] - qregf[qi][l];
+                                    accv[qi] = __builtin_fmaf(d, d, accv[qi]);
+                                } else {
+                                    const double d = ring[p][l] - qreg[qi][l];
+                                    accv[qi] = __builtin_fma(d, d, accv[qi]);
                                 }
-                                hit = e <= T;
                             }
-                            m |= __builtin_amdgcn_ballot_w64(hit);
-                            my_band += (uint64_t)__popcll(mb);
                         }
+#pragma unroll
+                        for (int qi = 0; qi < kQB; qi++) __asm__ volatile("" ::"v"(accv[qi]) : "memory");
+                        load((c0 + p + kDepth) * 64, p);
+#pragma unroll
+                        for (int qi = 0; qi < kQB; qi++) {
+                            const uint64_t vq = (uint32_t)qi < nqg ? vmask : 0ull;
+                            uint64_t m;
+                            if constexpr (F32) {
+                                m = __builtin_amdgcn_ballot_w64(accv[qi] <= a.tacc32) & vq;
+                                const uint64_t mb = vq & ~m & ~__builtin_amdgcn_ballot_w64(accv[qi] > a.trej32);
+                                if (mb) {
+                                    // the band: the exact FP64 chain decides
+                                    bool hit = false;
+                                    if (__builtin_amdgcn_inverse_ballot_w64(mb)) {
+                                        double e = 0.0;
+#pragma unroll
+                                        for (int l = 0; l < DQ; l++) {
+                                            const double d = soa[(uint64_t)l * n + pos] - a.queries[(uint64_t)qid[qi] * dim + l];
+                                            e = __builtin_fma(d, d, e);
+                                        }
+                                        hit = e <= T;
+                                    }
+                                    m |= __builtin_amdgcn_ballot_w64(hit);
+                                    my_band += (uint64_t)__popcll(mb);
+                                }
+                            } else {
+                                m = __builtin_amdgcn_ballot_w64(accv[qi] <= T) & vq;
+                            }
+                            if (FILL && m && __builtin_amdgcn_inverse_ballot_w64(m))
+                                a.out[base[qi] + cnt[qi] + lanes_below(m)] = pid;
+                            cnt[qi] += (uint32_t)__popcll(m);
+                        }
+                    }
+                }
+            } else {
+                for (uint32_t t0 = 0; t0 < M; t0 += 64) {
+                    const uint32_t t = t0 + lane;
+                    const bool valid = t < M;
+                    uint32_t pos = 0;
+                    if (valid) {
+                        while (slot + 1 < nslots && rl.off[slot + 1] <= t) slot++;
+                        pos = rl.start[slot] + (t - rl.off[slot]);
+                    }
+                    // one compare per ballot, validity applied to the scalar
+                    // masks; the hit lanes of a fill store under the mask itself
+                    const uint64_t vmask = __builtin_amdgcn_ballot_w64(valid);
+                    uint32_t pid = 0;
+                    if (FILL) pid = a.ix.ila[valid ? pos : 0u];
+                    double acc[kQB];
+    #pragma unroll
+                    for (int qi = 0; qi < kQB; qi++) acc[qi] = 0.0;
+                    if constexpr (F32) {
+                        float x[DQ];
+                        if constexpr (DQ % 2 == 0) {
+                            typedef float f2 __attribute__((ext_vector_type(2)));
+                            const f2 *src = reinterpret_cast<const f2 *>(a.aos32 + (uint64_t)pos * DQ);
+    #pragma unroll
+                            for (int l = 0; l < DQ / 2; l++) {
+                                const f2 v = src[l];
+                                x[2 * l] = v.x;
+                                x[2 * l + 1] = v.y;
+                            }
+                        } else {
+    #pragma unroll
+                            for (int l = 0; l < DQ; l++) x[l] = a.soa32[(uint64_t)l * n + pos];
+                        }
+    #pragma unroll
+                        for (int qi = 0; qi < kQB; qi++) {
+                            float accf = 0.0f;
+    #pragma unroll
+                            for (int l = 0; l < DQ; l++) {
+                                const float d = x[l] - qregf[qi][l];
+                                accf = __builtin_fmaf(d, d, accf);
+                            }
+                            const uint64_t vq = (uint32_t)qi < nqg ? vmask : 0ull;
+                            uint64_t m = __builtin_amdgcn_ballot_w64(accf <= a.tacc32) & vq;
+                            const uint64_t mb = vq & ~m & ~__builtin_amdgcn_ballot_w64(accf > a.trej32);
+                            if (mb) {
+                                // the band: the exact FP64 chain decides
+                                bool hit = false;
+                                if (__builtin_amdgcn_inverse_ballot_w64(mb)) {
+                                    double e = 0.0;
+    #pragma unroll
+                                    for (int l = 0; l < DQ; l++) {
+                                        const double d = soa[(uint64_t)l * n + pos] - a.queries[(uint64_t)qid[qi] * dim + l];
+                                        e = __builtin_fma(d, d, e);
+                                    }
+                                    hit = e <= T;
+                                }
+                                m |= __builtin_amdgcn_ballot_w64(hit);
+                                my_band += (uint64_t)__popcll(mb);
+                            }
+                            if (FILL && m && __builtin_amdgcn_inverse_ballot_w64(m))
+                                a.out[base[qi] + cnt[qi] + lanes_below(m)] = pid;
+                            cnt[qi] += (uint32_t)__popcll(m);
+                        }
+                        continue;
+                    } else if constexpr (D > 0) {
+                        double x[DQ];
+                        // unconditional (invalid lanes read position 0, masked below)
+                        if constexpr (D <= (int)kAosMaxDim) {
+                            const double *src = a.ix.aos + (uint64_t)pos * DQ;
+    #pragma unroll
+                            for (int l = 0; l < DQ; l++) x[l] = src[l];
+                        } else {
+    #pragma unroll
+                            for (int l = 0; l < DQ; l++) x[l] = soa[(uint64_t)l * n + pos];
+                        }
+    #pragma unroll
+                        for (int qi = 0; qi < kQB; qi++) {
+    #pragma unroll
+                            for (int l = 0; l < DQ; l++) {
+                                const double d = x[l] - qreg[qi][l];
+                                acc[qi] = __builtin_fma(d, d, acc[qi]);
+                            }
+                        }
+                    } else {
+                        for (uint32_t l = 0; l < dim; l++) {
+                            const double xl = valid ? soa[(uint64_t)l * n + pos] : 0.0;
+    #pragma unroll
+                            for (int qi = 0; qi < kQB; qi++) {
+                                const double d = xl - a.queries[(uint64_t)qid[qi] * dim + l];
+                                acc[qi] = __builtin_fma(d, d, acc[qi]);
+                            }
+                        }
+                    }
+    #pragma unroll
+                    for (int qi = 0; qi < kQB; qi++) {
+                        const uint64_t m = __builtin_amdgcn_ballot_w64(acc[qi] <= T) & ((uint32_t)qi < nqg ? vmask : 0ull);
                         if (FILL && m && __builtin_amdgcn_inverse_ballot_w64(m))
                             a.out[base[qi] + cnt[qi] + lanes_below(m)] = pid;
                         cnt[qi] += (uint32_t)__popcll(m);
                     }
-                    continue;
-                } else if constexpr (D > 0) {
-                    double x[DQ];
-                    // unconditional (invalid lanes read position 0, masked below)
-                    if constexpr (D <= (int)kAosMaxDim) {
-                        const double *src = a.ix.aos + (uint64_t)pos * DQ;
-#pragma unroll
-                        for (int l = 0; l < DQ; l++) x[l] = src[l];
-                    } else {
-#pragma unroll
-                        for (int l = 0; l < DQ; l++) x[l] = soa[(uint64_t)l * n + pos];
-                    }
-#pragma unroll
-                    for (int qi = 0; qi < kQB; qi++) {
-#pragma unroll
-                        for (int l = 0; l < DQ; l++) {
-                            const double d = x[l] - qreg[qi][l];
-                            acc[qi] = __builtin_fma(d, d, acc[qi]);
-                        }
-                    }
-                } else {
-                    for (uint32_t l = 0; l < dim; l++) {
-                        const double xl = valid ? soa[(uint64_t)l * n + pos] : 0.0;
-#pragma unroll
-                        for (int qi = 0; qi < kQB; qi++) {
-                            const double d = xl - a.queries[(uint64_t)qid[qi] * dim + l];
-                            acc[qi] = __builtin_fma(d, d, acc[qi]);
-                        }
-                    }
-                }
-#pragma unroll
-                for (int qi = 0; qi < kQB; qi++) {
-                    const uint64_t m = __builtin_amdgcn_ballot_w64(acc[qi] <= T) & ((uint32_t)qi < nqg ? vmask : 0ull);
-                    if (FILL && m && __builtin_amdgcn_inverse_ballot_w64(m))
-                        a.out[base[qi] + cnt[qi] + lanes_below(m)] = pid;
-                    cnt[qi] += (uint32_t)__popcll(m);
                 }
             }
             if (!FILL && lane < (int)nqg) {
@@ -334,7 +513,6 @@ __global__ __launch_bounds__(64 * kQWaves) void k_qjoin(QJoinArgs a) {
             }
             if (lane == 0) my_evals += (uint64_t)M * nqg;
         }
-        wave_lds_sync();  // the row list is rebuilt for the next unit
     }
     if (lane == 0) {
         if (!FILL && my_evals) atomicAdd(a.evals, (unsigned long long)my_evals);
