@@ -179,7 +179,7 @@ def attach_traffic(roof, config, refine, kernel_ms):
                                   f"this run {kernel_ms:.2f} ms)")
 
 
-def serve_levels(eps_lo, eps_hi, ratio=2 ** 0.25, max_levels=16):
+def serve_levels(eps_lo, eps_hi, ratio=2 ** 0.125, max_levels=32):
     """Grid levels for the serving index: cell sides from just above eps_hi
     down by `ratio` while they still exceed eps_lo / ratio, so a request's
     cell side is at most `ratio` times its eps."""

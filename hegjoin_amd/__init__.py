@@ -97,7 +97,7 @@ MODES = {"gpu": 0, "hybrid": 1, "static": 2, "cpu": 3}
 
 class _ServeOpts(C.Structure):
     _fields_ = [("dim", C.c_uint32), ("indexed_dims", C.c_uint32), ("n_levels", C.c_uint32),
-                ("slots", C.c_uint32), ("cell_sides", C.c_double * 16),
+                ("slots", C.c_uint32), ("cell_sides", C.c_double * 32),
                 ("result_bytes", C.c_uint64), ("max_queries", C.c_uint32), ("device", C.c_int32)]
 
 
@@ -554,8 +554,8 @@ class Server:
         o.dim = dim
         o.indexed_dims = indexed_dims if indexed_dims is not None else min(dim, 6)
         sides = list(cell_sides)
-        if not 1 <= len(sides) <= 16:
-            raise ValueError("1 to 16 cell sides")
+        if not 1 <= len(sides) <= len(o.cell_sides):
+            raise ValueError(f"1 to {len(o.cell_sides)} cell sides")
         o.n_levels = len(sides)
         for i, c in enumerate(sides):
             o.cell_sides[i] = float(c)

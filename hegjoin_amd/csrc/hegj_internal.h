@@ -268,8 +268,12 @@ struct QJoinArgs {
     const float *aos32;          // point-major copy (dim <= kAosMaxDim)
     const float *q32;            // [nq * dim] row-major, centred like soa32
     float tacc32, trej32;
-    unsigned long long *band;    // pairs rechecked in FP64 (fill pass)
+    unsigned long long *band;    // pairs rechecked in FP64 (count pass)
+    // [nq * kQueryHitWords] hit-chunk bitmaps: written by the count pass,
+    // read by the fill pass (null: the fill pass visits every chunk)
+    uint64_t *hitbits;
 };
+constexpr uint32_t kQueryHitWords = 64;
 hegj_status query_geom(const GridGeom &g, QueryGeom *qg);
 // q32 = fl32(q - centre); *rmax_bits = bits of max |q - centre| (a positive
 // double, ordered like its bit pattern; must be zeroed by the caller).

@@ -19,10 +19,12 @@
 //   k_qruns         one unit per run of equal keys (up to 16 queries of one
 //                   cell); each group of 4 of them builds its stencil rows,
 //                   pruned to the cells within eps of one of its queries;
-//   k_qjoin<count>  per-query neighbour counts;
+//   k_qjoin<count>  per-query neighbour counts, and per group a bitmap of the
+//                   candidate chunks in which one of its queries has a hit;
 //   scan            CSR offsets, exact result size (admission happens here);
 //   k_qjoin<fill>   each query's neighbours written at its own offset in
-//                   candidate (cell) order -- no atomics, deterministic.
+//                   candidate (cell) order, visiting only the hit chunks --
+//                   no atomics, deterministic.
 #include <type_traits>
 
 #include "hegj_internal.h"
@@ -32,12 +34,20 @@ namespace hegj {
 namespace {
 
 constexpr int kQWaves = 4;
-constexpr int kQB = 4;  // queries evaluated together against one candidate chunk
+// Queries evaluated together against one candidate chunk, on the FP64 path
+// (their coordinates fill 48 scalar registers at 6-D) and on the FP32 filter.
+// 8 on the FP32 filter measured slower at 6-D (10M points, 93k queries,
+// eps 0.0172): the group's pruned stencil grows (+12% distance evaluations)
+// and its registers spill.
+constexpr int kQB = 4;
+constexpr int kQBF32 = 4;
+constexpr int kMaxQB = 16;
+constexpr uint32_t kHitWords = kQueryHitWords;
 
 struct QRowList {
     uint32_t start[kMaxRows];
     uint32_t off[kMaxRows + 1];
-    float2 gap[kQB * kMaxK];  // per (query, indexed dim): squared gaps to the -1 / +1 cell
+    float2 gap[kMaxQB * kMaxK];  // per (query, indexed dim): squared gaps to the -1 / +1 cell
 };
 
 __global__ __launch_bounds__(256) void k_qcell_keys(const double *__restrict__ q, uint32_t nq,
@@ -112,6 +122,7 @@ __global__ __launch_bounds__(256) void k_sub_fill(const uint32_t *__restrict__ s
 // most T / side^2.  Gaps are shrunk by 1e-6 + 2e-15 (|c_j| + 2) (the FP64
 // rounding of the cell assignments) and the bound raised by 1e-5 relative
 // (FP32 sums), so the kept set can only be larger than the exact one.
+template <int QB>
 __device__ uint32_t qbuild_rows(const QJoinArgs &a, uint64_t key, const uint32_t *qid, uint32_t nqg,
                                 QRowList &rl, uint32_t *nslots) {
     const GridGeom &g = a.qg.g;
@@ -144,7 +155,7 @@ __device__ uint32_t qbuild_rows(const QJoinArgs &a, uint64_t key, const uint32_t
         uint32_t id = 0;
         int64_t cj = 0;
 #pragma unroll
-        for (int t = 0; t < kQB; t++) id = (uint32_t)t == qi ? qid[t] : id;
+        for (int t = 0; t < QB; t++) id = (uint32_t)t == qi ? qid[t] : id;
 #pragma unroll
         for (int jj = 0; jj < kMaxK; jj++) cj = (uint32_t)jj == j ? cs[jj] : cj;
         const double x = a.queries[(uint64_t)id * dim + j];
@@ -233,8 +244,11 @@ __device__ uint32_t qbuild_rows(const QJoinArgs &a, uint64_t key, const uint32_t
 // F32: candidates from the level's centred FP32 copy, queries from the
 // request's (centred the same way); acc32 <= tacc32 accepts, acc32 > trej32
 // rejects, the band between is decided by the exact FP64 chain.
-template <int D, bool FILL, bool F32 = false>
-__global__ __launch_bounds__(64 * kQWaves) void k_qjoin(QJoinArgs a) {
+// Held to 3 waves per SIMD (168 registers, a few spilled): at 2 waves the
+// candidate loads of too few waves are in flight (6-D, 10M points, 93k
+// queries: 290 -> 270 ms per request).
+template <int D, bool FILL, bool F32 = false, int QB = kQB>
+__global__ __launch_bounds__(64 * kQWaves) __attribute__((amdgpu_waves_per_eu(3))) void k_qjoin(QJoinArgs a) {
     static_assert(!F32 || D > 0, "FP32 filter needs a fixed dimensionality");
     __shared__ QRowList rls[kQWaves];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -251,49 +265,52 @@ __global__ __launch_bounds__(64 * kQWaves) void k_qjoin(QJoinArgs a) {
         if (u >= a.n_units) break;
         const uint32_t s0 = a.starts[u], s1 = a.starts[u + 1];
         const uint64_t ukey = a.keys[s0];
-        for (uint32_t g0 = s0; g0 < s1; g0 += kQB) {
-            const uint32_t nqg = min((uint32_t)kQB, s1 - g0);
-            uint32_t qid[kQB];
-            uint64_t base[kQB];
-            uint32_t cnt[kQB];
+        for (uint32_t g0 = s0; g0 < s1; g0 += QB) {
+            const uint32_t nqg = min((uint32_t)QB, s1 - g0);
+            uint32_t qid[QB];
+            uint64_t base[QB];
+            uint32_t cnt[QB];
 #pragma unroll
-            for (int qi = 0; qi < kQB; qi++) {
+            for (int qi = 0; qi < QB; qi++) {
                 qid[qi] = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.ids[g0 + min((uint32_t)qi, nqg - 1)]);
                 base[qi] = FILL ? a.offsets[qid[qi]] : 0;
                 cnt[qi] = 0;
             }
             uint32_t nslots;
-            const uint32_t M = qbuild_rows(a, ukey, qid, nqg, rl, &nslots);
+            const uint32_t M = qbuild_rows<QB>(a, ukey, qid, nqg, rl, &nslots);
             // the group's query coordinates, once (lane i loads element i;
             // scalar registers), so no load in the candidate loop waits on
             // anything but the candidates
             constexpr int DQ = D > 0 ? D : 1;
-            double qreg[F32 ? 1 : kQB][DQ];
-            float qregf[F32 ? kQB : 1][DQ];
+            double qreg[F32 ? 1 : QB][DQ];
+            float qregf[F32 ? QB : 1][DQ];
             if constexpr (F32) {
-                const int qi = lane / DQ, l = lane % DQ;
-                float v = 0.0f;
-                if (lane < kQB * DQ) {
-                    uint32_t id = 0;
 #pragma unroll
-                    for (int k = 0; k < kQB; k++) id = k == qi ? qid[k] : id;
-                    v = a.q32[(uint64_t)id * dim + l];
+                for (int h = 0; h < (QB * DQ + 63) / 64; h++) {
+                    const int e = h * 64 + lane, qi = e / DQ, l = e % DQ;
+                    float v = 0.0f;
+                    if (e < QB * DQ) {
+                        uint32_t id = 0;
+#pragma unroll
+                        for (int k = 0; k < QB; k++) id = k == qi ? qid[k] : id;
+                        v = a.q32[(uint64_t)id * dim + l];
+                    }
+#pragma unroll
+                    for (int j = h * 64; j < QB * DQ && j < h * 64 + 64; j++)
+                        qregf[j / DQ][j % DQ] = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(v), j - h * 64));
                 }
-#pragma unroll
-                for (int j = 0; j < kQB * DQ; j++)
-                    qregf[j / DQ][j % DQ] = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(v), j));
             } else if constexpr (D > 0) {
-                static_assert(kQB * D <= 64, "one load per lane");
+                static_assert(QB * D <= 64, "one load per lane");
                 const int qi = lane / DQ, l = lane % DQ;
                 double v = 0.0;
-                if (lane < kQB * DQ) {
+                if (lane < QB * DQ) {
                     uint32_t id = 0;
 #pragma unroll
-                    for (int k = 0; k < kQB; k++) id = k == qi ? qid[k] : id;
+                    for (int k = 0; k < QB; k++) id = k == qi ? qid[k] : id;
                     v = a.queries[(uint64_t)id * dim + l];
                 }
 #pragma unroll
-                for (int j = 0; j < kQB * DQ; j++) {
+                for (int j = 0; j < QB * DQ; j++) {
                     const uint64_t u = (uint64_t)__double_as_longlong(v);
                     const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, j);
                     const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), j);
@@ -343,20 +360,50 @@ __global__ __launch_bounds__(64 * kQWaves) void k_qjoin(QJoinArgs a) {
                     }
                     if (FILL) pidr[p] = a.ix.ila[pos];
                 };
-#pragma unroll
-                for (int p = 0; p < kDepth; p++) load((uint32_t)p * 64, p);
+                // Hit-chunk bitmap of the group (a.hitbits: kHitWords words
+                // per query, the group's at its first query), one bit per
+                // `span` chunks.  The count pass sets the bits of chunks in
+                // which some query of the group has a neighbour; the fill
+                // pass, which does the same arithmetic, visits only those.
                 const uint32_t nch = (M + 63) / 64;
-                for (uint32_t c0 = 0; c0 < nch; c0 += kDepth) {
+                uint64_t *hb = a.hitbits ? a.hitbits + (uint64_t)g0 * kHitWords : nullptr;
+                const uint32_t capb = nqg * kHitWords * 64;
+                const uint32_t span = nch > capb ? (nch + capb - 1) / capb : 1u;
+                const uint32_t nw = ((nch + span - 1) / span + 63) / 64;
+                uint32_t cur = 0, cend = 0, wi = 0, bitpos = 0, subc = 0;
+                uint64_t w = (FILL && hb && nw > 0) ? hb[0] : 0ull, hitw = 0;
+                // next chunk to visit (nch when none is left)
+                auto next = [&]() __attribute__((always_inline)) -> uint32_t {
+                    if (!FILL || !hb) return cur < nch ? cur++ : nch;
+                    if (cur + 1 < cend) return ++cur;
+                    while (w == 0) {
+                        if (++wi >= nw) return cur = cend = nch;
+                        w = hb[wi];
+                    }
+                    const uint32_t b = wi * 64 + (uint32_t)__builtin_ctzll(w);
+                    w &= w - 1;
+                    cur = b * span;
+                    cend = min(cur + span, nch);
+                    return cur;
+                };
+                uint32_t chr[kDepth];
+#pragma unroll
+                for (int p = 0; p < kDepth; p++) {
+                    chr[p] = next();
+                    load(chr[p] * 64, p);
+                }
+                while (chr[0] < nch) {
 #pragma unroll
                     for (int p = 0; p < kDepth; p++) {
-                        const uint32_t t0 = (c0 + p) * 64;
+                        const uint32_t ch = chr[p];
+                        const uint32_t t0 = ch * 64;
                         const bool valid = t0 + lane < M;
                         const uint64_t vmask = __builtin_amdgcn_ballot_w64(valid);
                         const uint32_t pos = posr[p];
                         const uint32_t pid = FILL ? pidr[p] : 0u;
-                        CT accv[kQB];
+                        CT accv[QB];
 #pragma unroll
-                        for (int qi = 0; qi < kQB; qi++) {
+                        for (int qi = 0; qi < QB; qi++) {
                             accv[qi] = (CT)0;
 #pragma unroll
                             for (int l = 0; l < DQ; l++) {
@@ -370,10 +417,12 @@ __global__ __launch_bounds__(64 * kQWaves) void k_qjoin(QJoinArgs a) {
                             }
                         }
 #pragma unroll
-                        for (int qi = 0; qi < kQB; qi++) __asm__ volatile("" ::"v"(accv[qi]) : "memory");
-                        load((c0 + p + kDepth) * 64, p);
+                        for (int qi = 0; qi < QB; qi++) __asm__ volatile("" ::"v"(accv[qi]) : "memory");
+                        chr[p] = next();
+                        load(chr[p] * 64, p);
+                        uint64_t anyhit = 0;
 #pragma unroll
-                        for (int qi = 0; qi < kQB; qi++) {
+                        for (int qi = 0; qi < QB; qi++) {
                             const uint64_t vq = (uint32_t)qi < nqg ? vmask : 0ull;
                             uint64_t m;
                             if constexpr (F32) {
@@ -400,9 +449,21 @@ __global__ __launch_bounds__(64 * kQWaves) void k_qjoin(QJoinArgs a) {
                             if (FILL && m && __builtin_amdgcn_inverse_ballot_w64(m))
                                 a.out[base[qi] + cnt[qi] + lanes_below(m)] = pid;
                             cnt[qi] += (uint32_t)__popcll(m);
+                            anyhit |= m;
+                        }
+                        if (!FILL && hb && ch < nch) {
+                            hitw |= (uint64_t)(anyhit != 0) << (bitpos & 63);
+                            if (++subc == span) {
+                                subc = 0;
+                                if ((++bitpos & 63) == 0) {
+                                    if (lane == 0) hb[(bitpos >> 6) - 1] = hitw;
+                                    hitw = 0;
+                                }
+                            }
                         }
                     }
                 }
+                if (!FILL && hb && ((bitpos & 63) != 0 || subc != 0) && lane == 0) hb[bitpos >> 6] = hitw;
             } else {
                 for (uint32_t t0 = 0; t0 < M; t0 += 64) {
                     const uint32_t t = t0 + lane;
@@ -417,9 +478,9 @@ __global__ __launch_bounds__(64 * kQWaves) void k_qjoin(QJoinArgs a) {
                     const uint64_t vmask = __builtin_amdgcn_ballot_w64(valid);
                     uint32_t pid = 0;
                     if (FILL) pid = a.ix.ila[valid ? pos : 0u];
-                    double acc[kQB];
+                    double acc[QB];
     #pragma unroll
-                    for (int qi = 0; qi < kQB; qi++) acc[qi] = 0.0;
+                    for (int qi = 0; qi < QB; qi++) acc[qi] = 0.0;
                     if constexpr (F32) {
                         float x[DQ];
                         if constexpr (DQ % 2 == 0) {
@@ -436,7 +497,7 @@ __global__ __launch_bounds__(64 * kQWaves) void k_qjoin(QJoinArgs a) {
                             for (int l = 0; l < DQ; l++) x[l] = a.soa32[(uint64_t)l * n + pos];
                         }
     #pragma unroll
-                        for (int qi = 0; qi < kQB; qi++) {
+                        for (int qi = 0; qi < QB; qi++) {
                             float accf = 0.0f;
     #pragma unroll
                             for (int l = 0; l < DQ; l++) {
@@ -478,7 +539,7 @@ __global__ __launch_bounds__(64 * kQWaves) void k_qjoin(QJoinArgs a) {
                             for (int l = 0; l < DQ; l++) x[l] = soa[(uint64_t)l * n + pos];
                         }
     #pragma unroll
-                        for (int qi = 0; qi < kQB; qi++) {
+                        for (int qi = 0; qi < QB; qi++) {
     #pragma unroll
                             for (int l = 0; l < DQ; l++) {
                                 const double d = x[l] - qreg[qi][l];
@@ -489,14 +550,14 @@ __global__ __launch_bounds__(64 * kQWaves) void k_qjoin(QJoinArgs a) {
                         for (uint32_t l = 0; l < dim; l++) {
                             const double xl = valid ? soa[(uint64_t)l * n + pos] : 0.0;
     #pragma unroll
-                            for (int qi = 0; qi < kQB; qi++) {
+                            for (int qi = 0; qi < QB; qi++) {
                                 const double d = xl - a.queries[(uint64_t)qid[qi] * dim + l];
                                 acc[qi] = __builtin_fma(d, d, acc[qi]);
                             }
                         }
                     }
     #pragma unroll
-                    for (int qi = 0; qi < kQB; qi++) {
+                    for (int qi = 0; qi < QB; qi++) {
                         const uint64_t m = __builtin_amdgcn_ballot_w64(acc[qi] <= T) & ((uint32_t)qi < nqg ? vmask : 0ull);
                         if (FILL && m && __builtin_amdgcn_inverse_ballot_w64(m))
                             a.out[base[qi] + cnt[qi] + lanes_below(m)] = pid;
@@ -507,7 +568,7 @@ __global__ __launch_bounds__(64 * kQWaves) void k_qjoin(QJoinArgs a) {
             if (!FILL && lane < (int)nqg) {
                 uint32_t mine = 0;
 #pragma unroll
-                for (int qi = 0; qi < kQB; qi++)
+                for (int qi = 0; qi < QB; qi++)
                     if (qi == lane) mine = cnt[qi];
                 a.counts[qid[lane]] = mine;
             }
@@ -516,7 +577,7 @@ __global__ __launch_bounds__(64 * kQWaves) void k_qjoin(QJoinArgs a) {
     }
     if (lane == 0) {
         if (!FILL && my_evals) atomicAdd(a.evals, (unsigned long long)my_evals);
-        if (FILL && my_band && a.band) atomicAdd(a.band, (unsigned long long)my_band);
+        if (!FILL && my_band && a.band) atomicAdd(a.band, (unsigned long long)my_band);
     }
 }
 
@@ -563,7 +624,7 @@ template <int D, bool FILL>
 hegj_status launch_q(const QJoinArgs &a, uint32_t blocks, hipStream_t s) {
     if constexpr (D > 0) {
         if (a.f32)
-            k_qjoin<D, FILL, true><<<blocks, 64 * kQWaves, 0, s>>>(a);
+            k_qjoin<D, FILL, true, kQBF32><<<blocks, 64 * kQWaves, 0, s>>>(a);
         else
             k_qjoin<D, FILL><<<blocks, 64 * kQWaves, 0, s>>>(a);
     } else {

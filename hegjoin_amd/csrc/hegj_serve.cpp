@@ -173,6 +173,8 @@ hegj_status hegj_server::run(Request *r, hipStream_t s, Arena &arena, void *ctr_
     HIP_TRY(keep.get(&q, (size_t)nq * dim));
     HIP_TRY(keep.get(&counts, nq));
     HIP_TRY(keep.get(&offsets, (size_t)nq + 1));
+    uint64_t *hitbits;
+    HIP_TRY(keep.get(&hitbits, (size_t)nq * kQueryHitWords));
     HIP_TRY(hipMemcpyAsync(q, r->queries.data(), sizeof(double) * nq * dim, hipMemcpyHostToDevice, s));
     QueryPlan plan;
     HEGJ_TRY(query_plan(&plan, keep, qg, q, nq, s));
@@ -214,6 +216,7 @@ hegj_status hegj_server::run(Request *r, hipStream_t s, Arena &arena, void *ctr_
     a.hash = &ctr->hash;
     a.evals = &ctr->evals;
     a.band = &ctr->band;
+    a.hitbits = hitbits;
     if (f32) {
         a.f32 = 1;
         a.soa32 = lf32[r->level].soa32;
@@ -259,7 +262,7 @@ void hegj_server::worker(uint32_t slot) {
     if (select_device(device) != HEGJ_OK || hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
         init_err = "slot: cannot create a stream";
     // one request at a time: its queries and every temporary of the pipeline
-    arena.cap = (size_t)opts.max_queries * (opts.dim * 8 + 128) + (32u << 20);
+    arena.cap = (size_t)opts.max_queries * (opts.dim * 8 + 128 + kQueryHitWords * 8) + (32u << 20);
     if (init_err.empty() && hipMalloc(reinterpret_cast<void **>(&arena.base), arena.cap) != hipSuccess)
         init_err = "slot: cannot allocate its device arena";
     if (init_err.empty() && hipMalloc(&ctr, 256) != hipSuccess) init_err = "slot: cannot allocate counters";
@@ -336,7 +339,7 @@ hegj_status hegj_serve_open(const double *database, uint32_t n, const hegj_serve
     if (o->indexed_dims == 0 || o->indexed_dims > HEGJ_MAX_INDEXED_DIMS || o->indexed_dims > o->dim)
         return fail(HEGJ_ERR_INVALID_ARG, "indexed_dims must be in [1, min(dim, 6)]");
     if (o->n_levels == 0 || o->n_levels > HEGJ_SERVE_MAX_LEVELS)
-        return fail(HEGJ_ERR_INVALID_ARG, "n_levels must be in [1, 16]");
+        return fail(HEGJ_ERR_INVALID_ARG, "n_levels must be in [1, 32]");
     if (o->slots == 0 || o->slots > 16) return fail(HEGJ_ERR_INVALID_ARG, "slots must be in [1, 16]");
     if (o->max_queries == 0) return fail(HEGJ_ERR_INVALID_ARG, "max_queries must be >= 1");
     if (o->result_bytes < 4096) return fail(HEGJ_ERR_INVALID_ARG, "result_bytes must be >= 4096");

@@ -415,7 +415,7 @@ hegj_status hegj_result_copy_csr(const hegj_result *res, uint64_t *offsets, uint
  * another request.  A request whose result alone is larger than the pool
  * fails with HEGJ_ERR_CAPACITY.
  * ------------------------------------------------------------------------ */
-#define HEGJ_SERVE_MAX_LEVELS 16
+#define HEGJ_SERVE_MAX_LEVELS 32
 typedef struct hegj_server hegj_server;
 
 typedef struct {
