@@ -64,7 +64,7 @@ class _JoinOpts(C.Structure):
                 ("queries_per_unit", C.c_uint32), ("refine", C.c_uint32),
                 ("xcd_parts", C.c_uint32), ("units_per_grab", C.c_uint32),
                 ("part_by_block", C.c_uint32), ("use_graph", C.c_uint32),
-                ("no_prune", C.c_uint32)]
+                ("no_prune", C.c_uint32), ("block_waves", C.c_uint32), ("batches", C.c_uint32)]
 
 
 class _ResultInfo(C.Structure):
@@ -412,7 +412,8 @@ class GridIndex:
         return cells, opi, tot.value
 
     def selfjoin(self, result_capacity=0, unit_order=None, queries_per_unit=None, xcd_parts=None,
-                 refine=None, units_per_grab=None, part_by_block=0, use_graph=False, prune=True):
+                 refine=None, units_per_grab=None, part_by_block=0, use_graph=False, prune=True,
+                 block_waves=None, batches=None):
         """Options left at None take the tuned defaults for this
         dimensionality (tuning.tsv, written by hegjoin_amd.autotune).
         unit_order: "morton", "spatial" (linear cell id) or "workload"
@@ -423,9 +424,12 @@ class GridIndex:
         use_graph: keep the FP64 plan in the index and replay a captured
         hipGraph of the batch (repeated joins with the same options);
         prune=False evaluates every point of every stencil cell (the pair set
-        is the same either way)."""
+        is the same either way); block_waves: waves per workgroup of the
+        grid join (1, 2 or 4); batches: device result batches the estimated
+        result is split into (with result_capacity 0)."""
         o = self._join_opts(result_capacity, unit_order, queries_per_unit, xcd_parts, refine,
-                            units_per_grab, part_by_block, use_graph, prune)
+                            units_per_grab, part_by_block, use_graph, prune,
+                            block_waves=block_waves, batches=batches)
         r = _vp()
         _check(_lib.hegj_selfjoin(self._h, C.byref(o), C.byref(r)))
         return NeighborTable(r, self)
@@ -466,7 +470,7 @@ class GridIndex:
 
     def _join_opts(self, result_capacity=0, unit_order=None, queries_per_unit=None,
                    xcd_parts=None, refine=None, units_per_grab=None, part_by_block=0,
-                   use_graph=False, prune=True, grid_only=False):
+                   use_graph=False, prune=True, grid_only=False, block_waves=None, batches=None):
         o = _JoinOpts()
         _lib.hegj_join_opts_tuned(C.byref(o), self.dim)
         o.result_capacity = result_capacity
@@ -485,6 +489,10 @@ class GridIndex:
         o.part_by_block = part_by_block
         o.use_graph = 1 if use_graph else 0
         o.no_prune = 0 if prune else 1
+        if block_waves is not None:
+            o.block_waves = block_waves
+        if batches is not None:
+            o.batches = batches
         return o
 
 

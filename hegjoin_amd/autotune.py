@@ -9,14 +9,17 @@ share one streamed candidate tile), unit order, per-XCD unit ranges, units
 per queue grab, and at 16 dimensions and above the refine (FP64 chain or the
 split-BF16 MFMA guard band) -- times each configuration, and checks that all
 of them produce the same pair count and pair hash.  A configuration whose
-result differs is an error, not a candidate.
+result differs is an error, not a candidate.  A second stage sweeps, around
+the first stage's best grid-join configuration, the workgroup size (1, 2 or
+4 waves: the reference's BLOCKSIZE, params.h) and the number of device
+result batches (1, 2, 4, 8: the reference's batch count, GPU.cu).
 
 The fastest configuration (host wall time of hegj_selfjoin, mean over
 --steps runs; among those within 1% of it, the one with the fastest join
 kernel) is written to tuning.tsv (one line per dimensionality: dim
-queries_per_unit unit_order xcd_parts units_per_grab refine), which the
-library, the Python package and the CLI load at start; every measurement goes
-to tuning.json.
+queries_per_unit unit_order xcd_parts units_per_grab refine block_waves
+batches), which the library, the Python package and the CLI load at start;
+every measurement goes to tuning.json.
 """
 from __future__ import annotations
 
@@ -56,6 +59,21 @@ def candidates(dim, with_fp64=False):
     return out
 
 
+WAVES = (4, 2, 1)
+BATCHES = (1, 2, 4, 8)
+
+
+def second_stage(best):
+    """Workgroup sizes and batch counts around a grid-join configuration (the
+    MFMA refine has its own fixed tiling and one pair list)."""
+    if best.get("refine") == "mfma":
+        return []
+    base = {k: best[k] for k in ("refine", "unit_order", "queries_per_unit", "xcd_parts",
+                                 "units_per_grab")}
+    return [dict(base, block_waves=w, batches=b) for w in WAVES for b in BATCHES
+            if (w, b) != (4, 1)]
+
+
 def measure(ix, opts, steps, limit_s):
     """Mean wall ms of hegj_selfjoin over `steps` runs (after one warm-up),
     with the result's pair count and hash.  Stops early past limit_s."""
@@ -79,8 +97,9 @@ def tune(dim, n, eps, k=None, steps=2, limit_s=20.0, log=print, with_fp64=False)
     k = k or min(dim, 6)
     results = []
     ref = None
-    with H.GridIndex(pts, eps, indexed_dims=k) as ix:
-        for opts in candidates(dim, with_fp64):
+    def run(configs):
+        nonlocal ref
+        for opts in configs:
             try:
                 ms, info, runs = measure(ix, opts, steps, limit_s)
             except H.HegjError as e:
@@ -91,10 +110,15 @@ def tune(dim, n, eps, k=None, steps=2, limit_s=20.0, log=print, with_fp64=False)
                 ref = key
             if key != ref:
                 raise RuntimeError(f"configuration {opts} changed the result: {key} != {ref}")
-            row = dict(opts, ms=ms, runs=runs, kernel_ms=info["join_kernel_avg_ms"],
+            row = dict({"block_waves": 4, "batches": 1}, **opts, ms=ms, runs=runs,
+                       kernel_ms=info["join_kernel_avg_ms"], batches_run=info["n_batches"],
                        pairs=info["total_neighbors"], pair_hash="%016x" % info["pair_hash"])
             results.append(row)
             log(json.dumps(row))
+
+    with H.GridIndex(pts, eps, indexed_dims=k) as ix:
+        run(candidates(dim, with_fp64))
+        run(second_stage(pick_best(results)))
     return pick_best(results), results
 
 
@@ -117,12 +141,13 @@ def write_tables(dim, workload, best, results, tsv=None, js=None):
     table[str(dim)] = dict(workload=workload, best=best, results=results)
     with open(js, "w") as f:
         json.dump(table, f, indent=1, sort_keys=True)
-    lines = ["# dim queries_per_unit unit_order xcd_parts units_per_grab refine",
+    lines = ["# dim queries_per_unit unit_order xcd_parts units_per_grab refine block_waves batches",
              "# written by hegjoin_amd/autotune.py; workloads and timings in tuning.json"]
     for d in sorted(table, key=int):
         b = table[d]["best"]
         lines.append(f"{d} {b['queries_per_unit']} {ORDERS[b['unit_order']]} {b['xcd_parts']} "
-                     f"{b['units_per_grab']} {REFINES[b['refine']]}")
+                     f"{b['units_per_grab']} {REFINES[b['refine']]} {b.get('block_waves', 4)} "
+                     f"{b.get('batches', 1)}")
     with open(tsv, "w") as f:
         f.write("\n".join(lines) + "\n")
     return tsv, js

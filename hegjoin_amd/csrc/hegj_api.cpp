@@ -400,6 +400,7 @@ struct JoinPlan {
     uint64_t *gmask = nullptr;
     uint32_t gmask_slots = 8192;
     uint32_t max_blocks = 0;
+    uint32_t waves = 4;  // waves per workgroup
     uint32_t parts = 1;
     uint32_t part_end[kMaxParts] = {};
     uint64_t capacity = 0;
@@ -421,14 +422,27 @@ struct JoinPlan {
 
 // About 32 grabs per wave keeps the queue atomics off the critical path.
 uint32_t grab_for(const JoinPlan &P, uint32_t nu) {
-    const uint64_t waves = (uint64_t)P.max_blocks * 4;
+    const uint64_t waves = (uint64_t)P.max_blocks * P.waves;
     const uint64_t g = nu / (waves * 32);
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, 64));
 }
 
+bool valid_waves_batches(const hegj_join_opts &o) {
+    return (o.block_waves == 0 || o.block_waves == 1 || o.block_waves == 2 || o.block_waves == 4) &&
+           o.batches <= 64;
+}
+
 uint32_t blocks_for(const JoinPlan &P, uint32_t nu, uint32_t grab) {
-    const uint64_t need = ((uint64_t)nu + grab * 4 - 1) / (grab * 4);
+    const uint64_t need = ((uint64_t)nu + grab * P.waves - 1) / (grab * P.waves);
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(need, P.max_blocks));
+}
+
+// Workgroups of `block_waves` waves: as many waves resident as with 4-wave
+// workgroups (occupancy is set by registers; workgroups that LDS keeps off a
+// CU wait and find the queue drained), so the block count scales by 4 / waves.
+void set_waves(JoinPlan &P, const hegj_join_opts &opts) {
+    P.waves = opts.block_waves ? opts.block_waves : 4u;
+    P.max_blocks = P.max_blocks * 4 / P.waves;
 }
 
 void set_parts(JoinArgs &x, uint32_t parts, const uint32_t *ends) {
@@ -452,6 +466,7 @@ JoinArgs base_args(const hegj_index *h, const hegj_join_opts &opts, const JoinPl
     a.deferred = P.deferred;
     a.gmask = P.gmask;
     a.gmask_slots = P.gmask_slots;
+    a.block_waves = P.waves;
     a.part_by_xcc = opts.part_by_block ? 0u : 1u;
     a.prune = opts.no_prune ? 0u : 1u;
     a.band = &P.d_ctr->band;
@@ -496,7 +511,8 @@ hegj_status make_plan(hegj_index *h, const hegj_join_opts &opts, uint32_t qb, co
     HEGJ_TRY(join_occupancy(h->g.dim, qb, h->device, &P.max_blocks,
                             P.f32 ? JoinKind::f32 : h->g.f32dt ? JoinKind::flt : JoinKind::fp64));
     // per-wave hit-mask scratch for units whose masks exceed LDS
-    HIP_TRY(dev_malloc_n(&P.gmask, (size_t)P.max_blocks * 4 * P.gmask_slots));
+    set_waves(P, opts);
+    HIP_TRY(dev_malloc_n(&P.gmask, (size_t)P.max_blocks * P.waves * P.gmask_slots));
     // unit ranges per XCD, split by estimated work
     P.parts = opts.xcd_parts ? std::min<uint32_t>(opts.xcd_parts, kMaxParts) : kMaxParts;
     if (U < P.parts * 256) P.parts = 1;
@@ -527,7 +543,8 @@ hegj_status make_plan(hegj_index *h, const hegj_join_opts &opts, uint32_t qb, co
         HIP_TRY(hipStreamSynchronize(s));
         const double est = (double)c.pairs * ((double)U / (double)ns);
         P.estimated_pairs = (uint64_t)est;
-        P.capacity = (uint64_t)(est * 1.10) + (1u << 20);
+        const uint32_t nb = opts.batches ? opts.batches : 1u;
+        P.capacity = (uint64_t)(est * 1.10 / nb) + (1u << 20);
         size_t free_b = 0, total_b = 0;
         HIP_TRY(hipMemGetInfo(&free_b, &total_b));
         P.capacity = std::min<uint64_t>(P.capacity, (uint64_t)(free_b * 0.6) / sizeof(uint32_t));
@@ -539,7 +556,8 @@ hegj_status make_plan(hegj_index *h, const hegj_join_opts &opts, uint32_t qb, co
 
 // Runs lists of work units into a result's batches.  A unit whose rows do
 // not fit in the open batch is deferred; deferred units go to a new batch,
-// sized exactly by a count-only pass over them.  A batch may stay open across
+// sized exactly by a count-only pass over them (at most the first batch's
+// size when the caller asked for several batches).  A batch may stay open across
 // run() calls (the hybrid modes feed the GPU its share in chunks): its pair
 // hash is taken when it closes, once every row it holds is written.  With
 // `last` the batch closes right after the launch (no further units will
@@ -680,7 +698,11 @@ struct BatchRunner {
             Counters c2;
             HIP_TRY(hipMemcpyAsync(&c2, P.d_ctr2, sizeof(c2), hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));
-            if (!opts.result_capacity) capacity = std::max<uint64_t>(c2.pairs, 1);
+            if (!opts.result_capacity) {
+                capacity = std::max<uint64_t>(c2.pairs, 1);
+                // a requested batch count keeps every batch at the first one's size
+                if (opts.batches > 1) capacity = std::min<uint64_t>(capacity, P.capacity);
+            }
         }
         return HEGJ_OK;
     }
@@ -933,7 +955,7 @@ hegj_status selfjoin_hybrid(hegj_index *h, const hegj_join_opts &opts, uint32_t 
 }  // namespace
 
 struct hegj::GraphPlan {
-    uint32_t key[9] = {};
+    uint32_t key[11] = {};
     JoinPlan P;
     uint32_t qb = 0, blocks = 0, grab = 1;
     JoinArgs *d_args = nullptr, *h_args = nullptr;
@@ -983,9 +1005,10 @@ hegj_status selfjoin_graph(hegj_index *h, const hegj_join_opts &opts, uint32_t q
                            const uint32_t *order, hegj_result *r, bool *fits) {
     hipStream_t s = h->stream;
     const uint32_t n = h->g.n;
-    const uint32_t key[9] = {qb, opts.unit_order, opts.xcd_parts, opts.units_per_grab,
-                             opts.part_by_block, (uint32_t)opts.result_capacity,
-                             (uint32_t)(opts.result_capacity >> 32), opts.refine, opts.no_prune};
+    const uint32_t key[11] = {qb, opts.unit_order, opts.xcd_parts, opts.units_per_grab,
+                              opts.part_by_block, (uint32_t)opts.result_capacity,
+                              (uint32_t)(opts.result_capacity >> 32), opts.refine, opts.no_prune,
+                              opts.block_waves, opts.batches};
     if (h->gplan && std::memcmp(h->gplan->key, key, sizeof(key)) != 0) {
         free_graph_plan(h->gplan);
         h->gplan = nullptr;
@@ -1342,6 +1365,7 @@ hegj_status hegj_tuning_set(uint32_t dim, const hegj_join_opts *opts) {
         return fail(HEGJ_ERR_INVALID_ARG, "queries_per_unit must be 0, 4, 8 or 16");
     if (opts->unit_order > HEGJ_ORDER_MORTON || opts->refine > HEGJ_REFINE_F32)
         return fail(HEGJ_ERR_INVALID_ARG, "unknown unit_order or refine");
+    if (!valid_waves_batches(*opts)) return fail(HEGJ_ERR_INVALID_ARG, "block_waves must be 0, 1, 2 or 4 and batches at most 64");
     std::lock_guard<std::mutex> lk(g_tuning_mu);
     g_tuning[dim] = *opts;
     return HEGJ_OK;
@@ -1358,11 +1382,12 @@ hegj_status hegj_tuning_load(const char *path, uint32_t *n_loaded) {
         lineno++;
         char *hash = std::strchr(line, '#');
         if (hash) *hash = 0;
-        unsigned dim, qb, order, parts, grab, refine;
-        const int got = std::sscanf(line, "%u %u %u %u %u %u", &dim, &qb, &order, &parts, &grab, &refine);
+        unsigned dim, qb, order, parts, grab, refine, waves = 0, batches = 0;
+        const int got = std::sscanf(line, "%u %u %u %u %u %u %u %u", &dim, &qb, &order, &parts, &grab, &refine,
+                                    &waves, &batches);
         if (got <= 0) continue;  // blank or comment
-        if (got != 6) {
-            st = fail(HEGJ_ERR_INVALID_ARG, "tuning table line " + std::to_string(lineno) + ": 6 fields expected");
+        if (got < 6) {
+            st = fail(HEGJ_ERR_INVALID_ARG, "tuning table line " + std::to_string(lineno) + ": 6 to 8 fields expected");
             break;
         }
         hegj_join_opts o;
@@ -1372,6 +1397,8 @@ hegj_status hegj_tuning_load(const char *path, uint32_t *n_loaded) {
         o.xcd_parts = parts;
         o.units_per_grab = grab;
         o.refine = refine;
+        o.block_waves = waves;
+        o.batches = batches;
         st = hegj_tuning_set(dim, &o);
         if (st != HEGJ_OK) break;
         count++;
@@ -1410,6 +1437,8 @@ hegj_status hegj_selfjoin(hegj_index *h, const hegj_join_opts *opts_in, hegj_res
     const uint32_t qb = opts.queries_per_unit ? opts.queries_per_unit : default_qb(h->g.dim);
     if (!mfma && qb != 4 && qb != 8 && qb != 16)
         return fail(HEGJ_ERR_INVALID_ARG, "queries_per_unit must be 4, 8 or 16");
+    if (!valid_waves_batches(opts))
+        return fail(HEGJ_ERR_INVALID_ARG, "block_waves must be 0, 1, 2 or 4 and batches at most 64");
     hipStream_t s = h->stream;
     const uint32_t n = h->g.n;
 
@@ -1493,6 +1522,8 @@ hegj_status hegj_selfjoin_noalloc(hegj_index *h, const hegj_join_opts *opts_in, 
     if (opts.refine > HEGJ_REFINE_F32) return fail(HEGJ_ERR_INVALID_ARG, "unknown refine");
     const uint32_t qb = opts.queries_per_unit ? opts.queries_per_unit : default_qb(h->g.dim);
     if (qb != 4 && qb != 8 && qb != 16) return fail(HEGJ_ERR_INVALID_ARG, "queries_per_unit must be 4, 8 or 16");
+    if (!valid_waves_batches(opts))
+        return fail(HEGJ_ERR_INVALID_ARG, "block_waves must be 0, 1, 2 or 4 and batches at most 64");
     // the index's lazily built arrays are inputs here: they must exist
     const char *prep = "the index must have run hegj_selfjoin once with these options first";
     if (!h->have_workload) return fail(HEGJ_ERR_INVALID_ARG, std::string(prep) + " (workload order)");
@@ -1558,6 +1589,7 @@ hegj_status hegj_selfjoin_noalloc(hegj_index *h, const hegj_join_opts *opts_in, 
         return too_small(HEGJ_ERR_OUT_OF_MEMORY);
     HEGJ_TRY(join_occupancy(h->g.dim, qb, h->device, &P.max_blocks,
                             P.f32 ? JoinKind::f32 : h->g.f32dt ? JoinKind::flt : JoinKind::fp64));
+    set_waves(P, opts);
     P.gmask_slots = 0;  // units whose masks exceed LDS re-evaluate in the write pass
     P.parts = opts.xcd_parts ? std::min<uint32_t>(opts.xcd_parts, kMaxParts) : kMaxParts;
     if (U < P.parts * 256) P.parts = 1;
@@ -1654,6 +1686,8 @@ hegj_status hegj_selfjoin_hybrid(hegj_index *h, const hegj_join_opts *opts_in,
     if (opts.refine > HEGJ_REFINE_F32) return fail(HEGJ_ERR_INVALID_ARG, "unknown refine");
     const uint32_t qb = opts.queries_per_unit ? opts.queries_per_unit : default_qb(h->g.dim);
     if (qb != 4 && qb != 8 && qb != 16) return fail(HEGJ_ERR_INVALID_ARG, "queries_per_unit must be 4, 8 or 16");
+    if (!valid_waves_batches(opts))
+        return fail(HEGJ_ERR_INVALID_ARG, "block_waves must be 0, 1, 2 or 4 and batches at most 64");
     std::unique_ptr<hegj_result> r(new hegj_result());
     r->idx = h;
     HEGJ_TRY(ensure_workload(h));

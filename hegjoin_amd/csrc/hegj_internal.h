@@ -143,6 +143,7 @@ struct JoinArgs {
     const uint2 *units;           // (cell, first query offset in cell)
     uint32_t n_units;
     uint32_t units_per_grab;
+    uint32_t block_waves;        // waves per workgroup of the launch (host side)
     uint32_t n_parts;             // unit ranges; a workgroup starts on its XCD's range
     uint32_t part_by_xcc;         // 1: range = hardware XCC id, 0: blockIdx % n_parts
     uint32_t part_begin[kMaxParts];

@@ -524,7 +524,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB>())) voi
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     RowList &rl = rls[wid];
     uint64_t *const hs_lds = hs_all[wid].w;
-    uint64_t *const hs_glb = a.gmask + (uint64_t)(blockIdx.x * kWavesPerBlock + wid) * a.gmask_slots;
+    uint64_t *const hs_glb = a.gmask + (uint64_t)(blockIdx.x * (blockDim.x >> 6) + wid) * a.gmask_slots;
     const double T = a.g.thresh;
     constexpr bool kRegs = (D > 0 && D <= kRegDims);
     constexpr int DR = kRegs ? D : 1;
@@ -943,7 +943,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, 4) void k_join_f32(JoinArgs av
     RowList &rl = rls[wid];
     F32Lds &fl = fls[wid];
     uint64_t *const hs_lds = hs_all[wid].w;
-    uint64_t *const hs_glb = a.gmask + (uint64_t)(blockIdx.x * kWavesPerBlock + wid) * a.gmask_slots;
+    uint64_t *const hs_glb = a.gmask + (uint64_t)(blockIdx.x * (blockDim.x >> 6) + wid) * a.gmask_slots;
     constexpr uint32_t kStride = D * (uint32_t)sizeof(float);
     constexpr uint32_t kChunkBytes = 64u * kStride;
     const uint32_t lane_off = (uint32_t)lane * kStride;
@@ -1383,6 +1383,12 @@ hegj_status build_units(const GridGeom &g, const DevIndex &ix, uint32_t qb, cons
     return HEGJ_OK;
 }
 
+// Threads per workgroup of a join launch: 64 per wave, `block_waves` waves
+// (the kernels index their per-wave LDS and mask slices by wave).
+static uint32_t block_threads(const JoinArgs &a) {
+    return 64u * (a.block_waves ? a.block_waves : (uint32_t)kWavesPerBlock);
+}
+
 template <int D, int QB>
 constexpr bool f32_inst() {
     return (D == 2 || D == 6) && QB * D <= 48;
@@ -1397,16 +1403,16 @@ static hegj_status launch_join_t(const JoinArgs &a, const JoinArgs *ap, uint32_t
     if (a.f32) {
         if constexpr (f32_inst<D, QB>()) {
             if (ap)
-                k_join_f32<D, QB, true><<<blocks, 64 * kWavesPerBlock, 0, s>>>(JoinArgs{}, ap);
+                k_join_f32<D, QB, true><<<blocks, block_threads(a), 0, s>>>(JoinArgs{}, ap);
             else
-                k_join_f32<D, QB, false><<<blocks, 64 * kWavesPerBlock, 0, s>>>(a, nullptr);
+                k_join_f32<D, QB, false><<<blocks, block_threads(a), 0, s>>>(a, nullptr);
         } else {
             return fail(HEGJ_ERR_UNSUPPORTED, "no FP32-filter kernel for this dim / queries_per_unit");
         }
     } else if (ap) {
-        k_join<D, QB, true><<<blocks, 64 * kWavesPerBlock, lds, s>>>(JoinArgs{}, ap);
+        k_join<D, QB, true><<<blocks, block_threads(a), lds, s>>>(JoinArgs{}, ap);
     } else {
-        k_join<D, QB, false><<<blocks, 64 * kWavesPerBlock, lds, s>>>(a, nullptr);
+        k_join<D, QB, false><<<blocks, block_threads(a), lds, s>>>(a, nullptr);
     }
     HIP_TRY(hipGetLastError());
     return HEGJ_OK;
@@ -1421,7 +1427,7 @@ static hegj_status launch_join_flt(const JoinArgs &a, const JoinArgs *ap, uint32
     const uint32_t lds = q_lds_bytes<0, QB>(a.g.dim);
     if (lds > kMaxQueryLds)
         return fail(HEGJ_ERR_UNSUPPORTED, "queries_per_unit * dim too large for LDS; lower queries_per_unit");
-    k_join<0, QB, false, true><<<blocks, 64 * kWavesPerBlock, lds, s>>>(a, nullptr);
+    k_join<0, QB, false, true><<<blocks, block_threads(a), lds, s>>>(a, nullptr);
     HIP_TRY(hipGetLastError());
     return HEGJ_OK;
 }

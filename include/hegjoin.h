@@ -218,6 +218,12 @@ typedef struct {
                                  (default): skip stencil cells whose box lies farther
                                  than eps from all queries of a work unit, with
                                  rounding margins; the pair set is the same */
+    uint32_t block_waves;     /* waves per workgroup of the FP64 and FP32-filter
+                                 joins: 1, 2 or 4; 0 = 4.  The reference's BLOCKSIZE
+                                 (params.h) is the thread count of its blocks */
+    uint32_t batches;         /* result_capacity 0: split the estimated result into
+                                 this many device batches (0 or 1 = one, up to 64);
+                                 the reference's batch count (GPU.cu) */
 } hegj_join_opts;
 
 void hegj_join_opts_default(hegj_join_opts *opts);
@@ -229,7 +235,8 @@ void hegj_join_opts_tuned(hegj_join_opts *opts, uint32_t dim);
 hegj_status hegj_tuning_set(uint32_t dim, const hegj_join_opts *opts);
 /* Load a tuning table written by hegjoin_amd/autotune.py: one line per
  * dimensionality, "dim queries_per_unit unit_order xcd_parts units_per_grab
- * refine", '#' starts a comment.  *n_loaded (may be NULL) = entries read. */
+ * refine [block_waves batches]", '#' starts a comment.  *n_loaded (may be
+ * NULL) = entries read. */
 hegj_status hegj_tuning_load(const char *path, uint32_t *n_loaded);
 
 hegj_status hegj_selfjoin(hegj_index *idx, const hegj_join_opts *opts, hegj_result **out);

@@ -57,7 +57,7 @@ int main(void) {
     O(hegj_hybrid_opts, gpu_fraction) O(hegj_hybrid_info, candidates_cpu) O(hegj_hybrid_info, cpu_end_s)
     O(hegj_result_info, estimated_pairs)
     O(hegj_index_info, build_ms) O(hegj_join_opts, part_by_block) O(hegj_join_opts, use_graph)
-    O(hegj_join_opts, no_prune)
+    O(hegj_join_opts, no_prune) O(hegj_join_opts, block_waves) O(hegj_join_opts, batches)
     O(hegj_result_info, refine_used) O(hegj_result_info, graph_replay) O(hegj_serve_opts, result_bytes)
     O(hegj_serve_opts, device) O(hegj_request_info, latency_ms) O(hegj_request_info, band_rechecks)
     O(hegj_request_info, refine_used) O(hegj_serve_stats, wall_ms)
@@ -210,13 +210,14 @@ def test_tuning_table_roundtrip(tmp_path):
     import hegjoin_amd as H
     from hegjoin_amd import autotune as T
     best = dict(refine="fp64", unit_order="workload", queries_per_unit=8, xcd_parts=1,
-                units_per_grab=4, ms=1.0)
+                units_per_grab=4, block_waves=2, batches=3, ms=1.0)
     tsv, js = T.write_tables(7, dict(dim=7), best, [best], str(tmp_path / "t.tsv"),
                              str(tmp_path / "t.json"))
     assert H.load_tuning(tsv) == 1
     o = H._JoinOpts()
     H.library().hegj_join_opts_tuned(C.byref(o), 7)
     assert (o.queries_per_unit, o.unit_order, o.xcd_parts, o.units_per_grab, o.refine) == (8, 1, 1, 4, 1)
+    assert (o.block_waves, o.batches) == (2, 3)
     H.library().hegj_join_opts_tuned(C.byref(o), 5)  # no entry: plain defaults
     assert (o.queries_per_unit, o.unit_order) == (0, 2)
     bad = tmp_path / "bad.tsv"
@@ -226,6 +227,14 @@ def test_tuning_table_roundtrip(tmp_path):
     bad.write_text("3 5 2 1 0 1\n")  # queries_per_unit 5 is not a kernel
     with pytest.raises(H.HegjError):
         H.load_tuning(str(bad))
+    bad.write_text("3 4 2 1 0 1 3 1\n")  # 3 waves per workgroup: not offered
+    with pytest.raises(H.HegjError):
+        H.load_tuning(str(bad))
+    six = tmp_path / "six.tsv"  # tables without the last two columns still load
+    six.write_text("3 4 2 1 0 1\n")
+    assert H.load_tuning(str(six)) == 1
+    H.library().hegj_join_opts_tuned(C.byref(o), 3)
+    assert (o.queries_per_unit, o.block_waves, o.batches) == (4, 0, 0)
     with pytest.raises(H.HegjError):
         H.load_tuning(str(tmp_path / "missing.tsv"))
 

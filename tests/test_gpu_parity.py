@@ -166,6 +166,32 @@ def test_join_random_vs_oracle(H, oracle, dim, k, n, eps, seed):
     assert info["pair_hash"] == oracle.csr_hash(eoff, enb)
 
 
+@pytest.mark.parametrize("refine", ["f32", "fp64"])
+def test_block_waves_and_batches_same_table(H, oracle, refine):
+    """Workgroups of 1, 2 or 4 waves and the estimated result split into up
+    to 8 device batches give the same neighbour table (6-D, 400k points,
+    ~20M pairs, so 8 batches are several batches of real size)."""
+    pts = oracle.generate_expo(400000, 6, seed=7)
+    eps = 0.0095
+    base = gpu_join(H, pts, eps, 6, refine=refine)
+    assert base[2]["refine_used"] == refine and base[2]["n_batches"] == 1
+    for w, b in [(1, 1), (2, 1), (4, 8), (2, 8), (1, 4)]:
+        print("case", refine, "block_waves", w, "batches", b, flush=True)
+        off, nb, info = gpu_join(H, pts, eps, 6, refine=refine, block_waves=w, batches=b)
+        if b > 1:
+            assert info["n_batches"] >= b // 2 + 1
+        assert info["pair_hash"] == base[2]["pair_hash"]
+        assert np.array_equal(off, base[0]) and np.array_equal(nb, base[1])
+
+
+def test_block_waves_and_batches_validated(H):
+    pts = np.random.default_rng(1).random((1000, 2))
+    with H.GridIndex(pts, 0.01, indexed_dims=2) as ix:
+        for kw in (dict(block_waves=3), dict(block_waves=8), dict(batches=65)):
+            with pytest.raises(H.HegjError):
+                ix.selfjoin(**kw)
+
+
 HIGH = golden_files("alg0_d18") + golden_files("alg0_d90")
 
 
@@ -322,16 +348,18 @@ def test_f32_filter_graph_replay(H, oracle):
             assert np.array_equal(off, eoff) and np.array_equal(nb, enb)
 
 
+@pytest.mark.parametrize("block_waves", [4, 1])
 @pytest.mark.parametrize("n,eps", [(40000, 0.004), (60000, 0.005)])
-def test_large_units_use_global_masks(H, oracle, n, eps):
+def test_large_units_use_global_masks(H, oracle, n, eps, block_waves):
     """Slabs (k = 1) holding thousands of points: with 16 queries per unit
     the (query, chunk) masks exceed LDS (first case: per-wave global
-    scratch) or even that scratch (second case, more than 8192 masks:
-    recomputed in the write phase)."""
+    scratch, one slice per wave of the launch at any workgroup size) or even
+    that scratch (second case, more than 8192 masks: recomputed in the write
+    phase)."""
     rng = np.random.default_rng(3)
     pts = rng.random((n, 3))
     pts[:, 0] *= 0.02
-    off, nb, info = gpu_join(H, pts, eps, 1, queries_per_unit=16)
+    off, nb, info = gpu_join(H, pts, eps, 1, queries_per_unit=16, block_waves=block_waves)
     eoff, enb = oracle.join(pts, 1, eps)
     assert np.array_equal(off, eoff) and np.array_equal(nb, enb)
 
@@ -559,7 +587,14 @@ def test_dtype_float_vs_oracle(H, oracle, dim, k, n, eps, refine, qb):
     eoff, enb = oracle.join(pts, k, eps, f32=True)
     assert np.array_equal(off, eoff) and np.array_equal(nb, enb)
     assert (ri["total_neighbors"], ri["pair_hash"]) == e32
-    assert ri["refine_used"] == ("f32" if (dim in (2, 6) and refine != "fp64"
+    # refine left open: the tuned default of this dimensionality decides
+    eff = refine
+    if eff is None:
+        import ctypes
+        o = H._JoinOpts()
+        H.library().hegj_join_opts_tuned(ctypes.byref(o), dim)
+        eff = {1: "fp64", 3: "f32"}.get(o.refine, "auto")
+    assert ri["refine_used"] == ("f32" if (dim in (2, 6) and eff != "fp64"
                                            and not (dim == 6 and qb == 16)) else "fp64")
     assert ri["band_rechecks"] == 0
 
