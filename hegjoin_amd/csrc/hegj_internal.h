@@ -466,6 +466,47 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
+// Per-lane cursor over a work unit's LDS slot list (start[s], off[s] for
+// s < nslots, off[nslots] = M, offsets strictly increasing): the lane's slot,
+// its position base start - off (mod 2^32) and the offset where the next
+// slot begins, so a step that stays inside the slot reads no LDS.  Calls of
+// pos() take t < M, never decreasing between calls.
+struct SlotCursor {
+    uint32_t slot, base, next;
+    template <typename L>
+    __device__ __forceinline__ void reset(const L &rl, uint32_t nslots = 0) {
+        (void)nslots;
+        slot = 0;
+        base = rl.start[0] - rl.off[0];
+        next = rl.off[1];
+    }
+    template <typename L>
+    __device__ __forceinline__ uint32_t pos(const L &rl, uint32_t t) {
+        while (next <= t) {
+            slot++;
+            base = rl.start[slot] - rl.off[slot];
+            next = rl.off[slot + 1];
+        }
+        return base + t;
+    }
+};
+
+// The same walk holding only the slot index (one register instead of three;
+// each step reads the next slot's offset and its own start from LDS).
+struct SlotWalk {
+    uint32_t slot, n;
+    template <typename L>
+    __device__ __forceinline__ void reset(const L &, uint32_t nslots) {
+        slot = 0;
+        n = nslots;
+    }
+    template <typename L>
+    __device__ __forceinline__ uint32_t pos(const L &rl, uint32_t t) {
+        while (slot + 1 < n && rl.off[slot + 1] <= t) slot++;
+        return rl.start[slot] + (t - rl.off[slot]);
+    }
+};
+
 inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 }  // namespace hegj

@@ -264,12 +264,6 @@ __device__ __forceinline__ uint32_t cand_pos(const RowList &rl, uint32_t nslots,
     return rl.start[lo] + (t - rl.off[lo]);
 }
 
-// Same, advancing a per-lane slot cursor (t never decreases for a lane).
-__device__ __forceinline__ uint32_t cand_pos_adv(const RowList &rl, uint32_t nslots, uint32_t t,
-                                                 uint32_t &slot) {
-    while (slot + 1 < nslots && rl.off[slot + 1] <= t) slot++;
-    return rl.start[slot] + (t - rl.off[slot]);
-}
 
 static_assert(HEGJ_MAX_POINTS < (1u << 29), "SoA byte offsets must fit 32 bits");
 
@@ -598,7 +592,11 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB>())) voi
             uint32_t cnt_l = 0;  // lane qi: hits of query qi so far
             uint64_t mine = 0;   // lane qi: query qi's mask of the current chunk
 
-            uint32_t slot = 0;
+            // the 4-wave instances (128 registers) keep the one-register walk:
+            // the cursor's two more registers spill there (6-D FP64: +4%)
+            typedef typename std::conditional<(join_min_waves<D, QB>() < 4), SlotCursor, SlotWalk>::type Cursor;
+            Cursor cur;
+            cur.reset(rl, nslots);
             // Candidate coordinates are prefetched kDepth chunks ahead in a
             // register ring (the loop is unrolled by kDepth so ring indices
             // are static): several chunks of loads stay in flight per wave.
@@ -609,7 +607,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB>())) voi
                 for (int p = 0; p < kDepth; p++) {
                     const uint32_t t = p * 64 + lane;
                     const bool v = t < M;
-                    load_cand<DR>(a, v ? cand_pos_adv(rl, nslots, t, slot) : 0u, v, ring[p]);
+                    load_cand<DR>(a, v ? cur.pos(rl, t) : 0u, v, ring[p]);
                 }
             }
             // Whole groups of kDepth chunks: the chunks past the end of the
@@ -652,10 +650,10 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB>())) voi
                                 }
                                 const uint32_t tn = (ch + kDepth) * 64 + lane;
                                 const bool vn = tn < M;
-                                load_cand<DR>(a, vn ? cand_pos_adv(rl, nslots, tn, slot) : 0u, vn, ring[p]);
+                                load_cand<DR>(a, vn ? cur.pos(rl, tn) : 0u, vn, ring[p]);
                             } else {
                                 const uint32_t t = ch * 64 + lane;
-                                const uint32_t pos = valid ? cand_pos_adv(rl, nslots, t, slot) : 0u;
+                                const uint32_t pos = valid ? cur.pos(rl, t) : 0u;
                                 uint32_t qoff = (uint32_t)wid * QB * dimq;
                                 __asm__ volatile("" : "+v"(qoff));
                                 dist_stream<D, QB, FLT>(a, pos, valid, q_dyn + qoff, nq, acc);
@@ -733,13 +731,14 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB>())) voi
             // -- are issued kW chunks ahead, so no iteration waits on a
             // fresh global load.
             constexpr int kW = QB <= 8 ? 8 : 4;
-            uint32_t slot2 = 0;
+            Cursor cur2;
+            cur2.reset(rl, nslots);
             if (keep) {
                 uint32_t idr[kW];
                 uint64_t mkr[kW];
                 auto fetch = [&](uint32_t c, uint32_t &id, uint64_t &mk) {
                     const uint32_t t = c * 64 + lane;
-                    id = gld(a.ix.ila + (t < M ? cand_pos_adv(rl, nslots, t, slot2) : 0u));
+                    id = gld(a.ix.ila + (t < M ? cur2.pos(rl, t) : 0u));
                     mk = (c < nchunks && (uint32_t)lane < nq) ? mask_get(c * nq + lane) : 0ull;
                 };
 #pragma unroll
@@ -782,7 +781,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, (join_min_waves<D, QB>())) voi
                 for (uint32_t ch = 0; ch < nchunks; ch++) {
                     const uint32_t t = ch * 64 + lane;
                     const bool valid = t < M;
-                    const uint32_t pos = valid ? cand_pos_adv(rl, nslots, t, slot2) : 0u;
+                    const uint32_t pos = valid ? cur2.pos(rl, t) : 0u;
                     const uint32_t id = gld(a.ix.ila + pos);
                     double x[DR];
                     if constexpr (kRegs) load_cand<DR>(a, pos, valid, x);
@@ -1247,11 +1246,12 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, 4) void k_join_f32(JoinArgs av
                                    (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)e_l, qi);
                 }
                 bool bad = false;
-                uint32_t slot2 = 0;
+                SlotWalk cur2;  // 128-register kernel: the one-register walk
+                cur2.reset(rl, nslots);
                 for (uint32_t ch = 0; ch < nchunks; ch++) {
                     const uint32_t t = ch * 64 + lane;
                     const bool valid = t < M;
-                    const uint32_t pos = valid ? cand_pos_adv(rl, nslots, t, slot2) : 0u;
+                    const uint32_t pos = valid ? cur2.pos(rl, t) : 0u;
                     const uint32_t id = gld(a.ix.ila + pos);
                     double x[D];
                     load_cand<D>(a, pos, valid, x);

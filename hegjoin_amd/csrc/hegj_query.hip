@@ -317,7 +317,8 @@ __global__ __launch_bounds__(64 * kQWaves) __attribute__((amdgpu_waves_per_eu(3)
                     qreg[j / DQ][j % DQ] = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
                 }
             }
-            uint32_t slot = 0;
+            SlotCursor cur;
+            cur.reset(rl);
             if constexpr (D > 0) {
                 // Candidate chunks kDepth ahead in a register ring (coordinates,
                 // positions and, when filling, point ids), so a wave keeps
@@ -330,11 +331,7 @@ __global__ __launch_bounds__(64 * kQWaves) __attribute__((amdgpu_waves_per_eu(3)
                 uint32_t posr[kDepth], pidr[kDepth];
                 auto load = [&](uint32_t t0, int p) __attribute__((always_inline)) {
                     const uint32_t t = t0 + lane;
-                    uint32_t pos = 0;
-                    if (t < M) {
-                        while (slot + 1 < nslots && rl.off[slot + 1] <= t) slot++;
-                        pos = rl.start[slot] + (t - rl.off[slot]);
-                    }
+                    const uint32_t pos = t < M ? cur.pos(rl, t) : 0u;
                     posr[p] = pos;
                     if constexpr (F32) {
                         if constexpr (DQ % 2 == 0) {
@@ -468,11 +465,7 @@ __global__ __launch_bounds__(64 * kQWaves) __attribute__((amdgpu_waves_per_eu(3)
                 for (uint32_t t0 = 0; t0 < M; t0 += 64) {
                     const uint32_t t = t0 + lane;
                     const bool valid = t < M;
-                    uint32_t pos = 0;
-                    if (valid) {
-                        while (slot + 1 < nslots && rl.off[slot + 1] <= t) slot++;
-                        pos = rl.start[slot] + (t - rl.off[slot]);
-                    }
+                    const uint32_t pos = valid ? cur.pos(rl, t) : 0u;
                     // one compare per ballot, validity applied to the scalar
                     // masks; the hit lanes of a fill store under the mask itself
                     const uint64_t vmask = __builtin_amdgcn_ballot_w64(valid);
