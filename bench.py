@@ -14,7 +14,10 @@ Prints one JSON line with value = result pairs/s (self pairs included, as the
 reference's [RESULT] count), the join kernel's roofline (live HIP events on
 the library's stream) and a CPU baseline: the reference's own Super-EGO
 (algorithm 3, Util.cpp / MultiThreadJoin.cpp compiled from source into
-oracle/_ref) timed on a seeded query sample on this host.
+oracle/_ref) timed on a seeded query sample on this host.  The default run
+also reports the other workloads (2-D, 18-D, 90-D, serving at 4 QPS), each
+run by this script in a child process, under `other_configs`
+(--no-others skips them).
 """
 from __future__ import annotations
 
@@ -306,6 +309,8 @@ def main():
     ap.add_argument("--n", type=int, default=0, help="override the point count (experiments)")
     ap.add_argument("--no-prune", action="store_true",
                     help="evaluate every point of every stencil cell (no cell-box pruning)")
+    ap.add_argument("--no-others", action="store_true",
+                    help="default run: skip the other workloads' summary (other_configs)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -398,7 +403,46 @@ def main():
         t0 = time.time()
         out["cpu_baseline"] = cpu_baseline(pts, eps, dim, a.cpu_seconds)
         print(f"# cpu baseline {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    if a.config == "6d" and not a.no_others and not a.n:
+        out["other_configs"] = other_configs()
     print(json.dumps(out))
+
+
+OTHER_RUNS = [
+    ("2d", ["--config", "2d"]),
+    ("18d", ["--config", "18d"]),
+    ("90d", ["--config", "90d"]),
+    ("serve_4qps", ["--config", "serve", "--qps", "4", "--requests", "30"]),
+]
+OTHER_KEYS = ("value", "unit", "ms_per_step", "join_kernel_ms", "refine", "pair_hash", "band_rechecks",
+              "distance_evals_per_s", "p50_ms", "p99_ms", "max_ms", "mean_ms", "completed", "rejected")
+
+
+def other_configs(timeout_s=400):
+    """The other workloads of the benchmark, each in a child process with its
+    own defaults (no CPU baseline): their key figures and roofline fraction,
+    so one default run reports every workload."""
+    import subprocess
+    res = {}
+    for key, args in OTHER_RUNS:
+        t0 = time.time()
+        cmd = [sys.executable, os.path.abspath(__file__), *args, "--no-cpu-baseline", "--no-others"]
+        try:
+            p = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout_s)
+            lines = [ln for ln in p.stdout.splitlines() if ln.startswith("{")]
+            if p.returncode != 0 or not lines:
+                res[key] = {"error": f"exit {p.returncode}", "stderr_tail": p.stderr[-400:]}
+            else:
+                d = json.loads(lines[-1])
+                r = {k: d[k] for k in OTHER_KEYS if k in d}
+                r["workload"] = d.get("config", {}).get("workload")
+                if d.get("roofline"):
+                    r["roofline"] = {k: d["roofline"].get(k) for k in ("bound", "achieved", "peak", "unit", "frac")}
+                res[key] = r
+        except subprocess.TimeoutExpired:
+            res[key] = {"error": f"timed out after {timeout_s} s"}
+        print(f"# {key}: {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    return res
 
 
 if __name__ == "__main__":
