@@ -429,7 +429,7 @@ uint32_t grab_for(const JoinPlan &P, uint32_t nu) {
 
 bool valid_waves_batches(const hegj_join_opts &o) {
     return (o.block_waves == 0 || o.block_waves == 1 || o.block_waves == 2 || o.block_waves == 4) &&
-           o.batches <= 64;
+           o.batches <= 32;  // leaves room under the 64-batch limit for deferrals
 }
 
 uint32_t blocks_for(const JoinPlan &P, uint32_t nu, uint32_t grab) {
@@ -1365,7 +1365,7 @@ hegj_status hegj_tuning_set(uint32_t dim, const hegj_join_opts *opts) {
         return fail(HEGJ_ERR_INVALID_ARG, "queries_per_unit must be 0, 4, 8 or 16");
     if (opts->unit_order > HEGJ_ORDER_MORTON || opts->refine > HEGJ_REFINE_F32)
         return fail(HEGJ_ERR_INVALID_ARG, "unknown unit_order or refine");
-    if (!valid_waves_batches(*opts)) return fail(HEGJ_ERR_INVALID_ARG, "block_waves must be 0, 1, 2 or 4 and batches at most 64");
+    if (!valid_waves_batches(*opts)) return fail(HEGJ_ERR_INVALID_ARG, "block_waves must be 0, 1, 2 or 4 and batches at most 32");
     std::lock_guard<std::mutex> lk(g_tuning_mu);
     g_tuning[dim] = *opts;
     return HEGJ_OK;
@@ -1438,7 +1438,7 @@ hegj_status hegj_selfjoin(hegj_index *h, const hegj_join_opts *opts_in, hegj_res
     if (!mfma && qb != 4 && qb != 8 && qb != 16)
         return fail(HEGJ_ERR_INVALID_ARG, "queries_per_unit must be 4, 8 or 16");
     if (!valid_waves_batches(opts))
-        return fail(HEGJ_ERR_INVALID_ARG, "block_waves must be 0, 1, 2 or 4 and batches at most 64");
+        return fail(HEGJ_ERR_INVALID_ARG, "block_waves must be 0, 1, 2 or 4 and batches at most 32");
     hipStream_t s = h->stream;
     const uint32_t n = h->g.n;
 
@@ -1523,7 +1523,7 @@ hegj_status hegj_selfjoin_noalloc(hegj_index *h, const hegj_join_opts *opts_in, 
     const uint32_t qb = opts.queries_per_unit ? opts.queries_per_unit : default_qb(h->g.dim);
     if (qb != 4 && qb != 8 && qb != 16) return fail(HEGJ_ERR_INVALID_ARG, "queries_per_unit must be 4, 8 or 16");
     if (!valid_waves_batches(opts))
-        return fail(HEGJ_ERR_INVALID_ARG, "block_waves must be 0, 1, 2 or 4 and batches at most 64");
+        return fail(HEGJ_ERR_INVALID_ARG, "block_waves must be 0, 1, 2 or 4 and batches at most 32");
     // the index's lazily built arrays are inputs here: they must exist
     const char *prep = "the index must have run hegj_selfjoin once with these options first";
     if (!h->have_workload) return fail(HEGJ_ERR_INVALID_ARG, std::string(prep) + " (workload order)");
@@ -1687,7 +1687,7 @@ hegj_status hegj_selfjoin_hybrid(hegj_index *h, const hegj_join_opts *opts_in,
     const uint32_t qb = opts.queries_per_unit ? opts.queries_per_unit : default_qb(h->g.dim);
     if (qb != 4 && qb != 8 && qb != 16) return fail(HEGJ_ERR_INVALID_ARG, "queries_per_unit must be 4, 8 or 16");
     if (!valid_waves_batches(opts))
-        return fail(HEGJ_ERR_INVALID_ARG, "block_waves must be 0, 1, 2 or 4 and batches at most 64");
+        return fail(HEGJ_ERR_INVALID_ARG, "block_waves must be 0, 1, 2 or 4 and batches at most 32");
     std::unique_ptr<hegj_result> r(new hegj_result());
     r->idx = h;
     HEGJ_TRY(ensure_workload(h));

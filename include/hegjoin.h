@@ -222,7 +222,7 @@ typedef struct {
                                  joins: 1, 2 or 4; 0 = 4.  The reference's BLOCKSIZE
                                  (params.h) is the thread count of its blocks */
     uint32_t batches;         /* result_capacity 0: split the estimated result into
-                                 this many device batches (0 or 1 = one, up to 64);
+                                 this many device batches (0 or 1 = one, up to 32);
                                  the reference's batch count (GPU.cu) */
 } hegj_join_opts;
 

@@ -187,7 +187,7 @@ def test_block_waves_and_batches_same_table(H, oracle, refine):
 def test_block_waves_and_batches_validated(H):
     pts = np.random.default_rng(1).random((1000, 2))
     with H.GridIndex(pts, 0.01, indexed_dims=2) as ix:
-        for kw in (dict(block_waves=3), dict(block_waves=8), dict(batches=65)):
+        for kw in (dict(block_waves=3), dict(block_waves=8), dict(batches=33)):
             with pytest.raises(H.HegjError):
                 ix.selfjoin(**kw)
 
