@@ -3,8 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (the HIP library, the CLI,
  * the Python package) links, loads or calls this code.  Only tests/, the
- * smoke() checker and the cpu_baseline leg of bench.py use it, and there only
- * as the checker / the timed CPU baseline.
+ * smoke() self-check and the cpu_baseline leg of bench.py use it, and there
+ * only as the expected result / the timed CPU baseline.
  *
  * It restates, in plain sequential C++ (OpenMP only over independent queries),
  * the reference's algorithm-0 semantics (grid extents, cell ids, grid index,

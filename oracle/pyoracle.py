@@ -1,7 +1,8 @@
 """ctypes wrapper for the CPU oracle (oracle/liboracle.so).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
-the checker and by bench.py's cpu_baseline leg.  The product never imports it.
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() for
+the expected result and by bench.py's cpu_baseline leg.  The product never
+imports it.
 
 Also drives the reference's own CPU code built from source into oracle/_ref
 (ref_harness_d<D>, gen_dataset) when those binaries exist.
