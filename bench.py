@@ -51,7 +51,7 @@ SERVE = dict(dim=6, n=10_000_000, k=6, nq_lo=1_000, nq_hi=500_000, eps_lo=0.002,
                   "log-uniform 1k-500k, eps log-uniform [0.002, 0.02]")
 PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector with packed (2-wide) FP32 instructions
-# This build compiles without packed FP32 (v_pk_*_f32), so an FP32 VALU
+# The library compiles without packed FP32 (v_pk_*_f32), so an FP32 VALU
 # instruction does one operation per lane: the roof it can reach is half the
 # packed peak.
 PEAK_FP32_UNPACKED_TFLOPS = PEAK_FP32_TFLOPS / 2
@@ -75,8 +75,8 @@ def calibrate(dim, n, target, ns=10000):
 
 def host_cpu():
     """CPU model, CPUs visible to this process, and the threads the baseline
-    uses: OMP_NUM_THREADS when set (a GPU box sets it to the host CPU share of
-    its one GPU), else every CPU the process may run on."""
+    uses: OMP_NUM_THREADS when set (a shared host sets it to the CPU share of
+    one GPU), else every CPU the process may run on."""
     model = "unknown"
     try:
         with open("/proc/cpuinfo") as f:
@@ -313,7 +313,7 @@ def main():
                     help="default run: skip the other workloads' summary (other_configs)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+        print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1",
                           "n_gpus": a.gpus}))
         sys.exit(2)
     if a.config == "serve":
