@@ -1094,7 +1094,8 @@ hegj_status ensure_gb(hegj_index *h) {
 // Join through the MFMA guard-band refine: one pass, the pair list sized on
 // the device, the band rechecked in FP64.  Same pair set as the FP64 kernel.
 // result_capacity (neighbour entries) sizes the first attempt's pair list.
-hegj_status selfjoin_mfma(hegj_index *h, const uint32_t *order, uint64_t result_capacity, hegj_result *r) {
+hegj_status selfjoin_mfma(hegj_index *h, const uint32_t *order, uint64_t result_capacity, uint32_t parts,
+                          hegj_result *r) {
     hipStream_t s = h->stream;
     const uint32_t n = h->g.n;
     HEGJ_TRY(ensure_gb(h));
@@ -1123,7 +1124,7 @@ hegj_status selfjoin_mfma(hegj_index *h, const uint32_t *order, uint64_t result_
     o.row_cnt = r->d_row_cnt;
     o.hash = d_hash;
     HEGJ_TRY(gb_join(h->g, h->ix, units, U, h->gb_A, h->gb_B, h->gb_w, h->gb_kp, h->device, s, &o,
-                     result_capacity));
+                     result_capacity, parts));
     r->batches.push_back(o.buf);
     r->bases.push_back(0);
     unsigned long long hash = 0;
@@ -1459,7 +1460,9 @@ hegj_status hegj_selfjoin(hegj_index *h, const hegj_join_opts *opts_in, hegj_res
         return fail(HEGJ_ERR_INVALID_ARG, "unknown unit_order");
     }
     if (mfma) {
-        HEGJ_TRY(selfjoin_mfma(h, order, opts.result_capacity, r.get()));
+        // xcd_parts 0 (auto): one unit range per XCD
+        HEGJ_TRY(selfjoin_mfma(h, order, opts.result_capacity, opts.xcd_parts ? opts.xcd_parts : kMaxParts,
+                               r.get()));
         r->info.total_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
         HEGJ_TRY(mark_done(r.get(), s));

@@ -231,10 +231,12 @@ hegj_status gb_prepare(const GridGeom &g, const DevIndex &ix, uint32_t kp, uint1
                        uint16_t **Bv, float **w, hipStream_t s);
 // pair_cap: size (8-byte keys) of the first attempt's pair list, 0 for the
 // default of 48 per point; an attempt that overflows is repeated once with a
-// list sized from the count it reached.
+// list sized from the count it reached.  parts: the units split into that
+// many contiguous ranges (1..kMaxParts), one queue per XCD.
 hegj_status gb_join(const GridGeom &g, const DevIndex &ix, const uint2 *units, uint32_t n_units,
                     const uint16_t *Av, const uint16_t *Bv, const float *w, uint32_t kp,
-                    int device, hipStream_t s, GbOutput *out, uint64_t pair_cap = 0);
+                    int device, hipStream_t s, GbOutput *out, uint64_t pair_cap = 0,
+                    uint32_t parts = 1);
 
 // ---- query join against a resident index (hegj_query.hip) ----------------
 struct QueryGeom {

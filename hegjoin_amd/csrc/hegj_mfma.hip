@@ -192,7 +192,9 @@ struct GbArgs {
     DevIndex ix;
     const uint2 *units;
     uint32_t n_units;
-    uint32_t *queue_head;
+    uint32_t *queue_heads;        // [kMaxParts] units taken per range
+    uint32_t n_parts;             // unit ranges: range p is [part_end[p - 1], part_end[p])
+    uint32_t part_end[kMaxParts];
     const uint16_t *Av;
     const uint16_t *Bv;
     const float *w;
@@ -279,8 +281,24 @@ __global__ __launch_bounds__(64 * gb_waves(KB)) void k_gb_join(GbArgs a) {
     Appender apA{0, 0}, apB{0, 0};
     uint64_t my_acc = 0, my_band = 0, my_evals = 0;
 
+    // A workgroup takes units from its XCD's range first (neighbouring cells
+    // share candidates in that XCD's L2), then from the other ranges.
+    const uint32_t part0 = xcc_id() % a.n_parts;
+    uint32_t tried = 0;  // ranges found empty (thread 0's)
     while (true) {
-        if (tid == 0) sh_unit[0] = atomicAdd(a.queue_head, 1u);
+        if (tid == 0) {
+            uint32_t got = ~0u;
+            while (tried < a.n_parts) {
+                const uint32_t p = (part0 + tried) % a.n_parts;
+                const uint32_t k = (p ? a.part_end[p - 1] : 0u) + atomicAdd(&a.queue_heads[p], 1u);
+                if (k < a.part_end[p]) {
+                    got = k;
+                    break;
+                }
+                tried++;
+            }
+            sh_unit[0] = got;
+        }
         __syncthreads();
         const uint32_t ui = sh_unit[0];
         if (ui >= a.n_units) break;
@@ -663,7 +681,7 @@ hegj_status gb_prepare(const GridGeom &g, const DevIndex &ix, uint32_t kp, uint1
 
 hegj_status gb_join(const GridGeom &g, const DevIndex &ix, const uint2 *units, uint32_t n_units,
                     const uint16_t *Av, const uint16_t *Bv, const float *w, uint32_t kp,
-                    int device, hipStream_t s, GbOutput *out, uint64_t pair_cap_first) {
+                    int device, hipStream_t s, GbOutput *out, uint64_t pair_cap_first, uint32_t parts) {
     const uint32_t kb = kp / 32;
     uint32_t max_blocks = 0;
     switch (kb) {
@@ -681,9 +699,11 @@ hegj_status gb_join(const GridGeom &g, const DevIndex &ix, const uint2 *units, u
                                        : std::max<uint64_t>((uint64_t)n * 48, 1u << 22);
     uint64_t band_cap = std::max<uint64_t>((uint64_t)n * 8, 1u << 22);
     struct Ctr {
-        uint32_t queue_head, overflow;
+        uint32_t queue_heads[kMaxParts], overflow, pad;
         unsigned long long pair_cursor, band_cursor, n_accept, n_band, evals;
     };
+    parts = std::max<uint32_t>(1, std::min<uint32_t>(parts, (uint32_t)kMaxParts));
+    if (n_units < parts * 64) parts = 1;
     Scratch sc;
     Ctr *d;
     HIP_TRY(sc.get(&d, 1));
@@ -699,7 +719,10 @@ hegj_status gb_join(const GridGeom &g, const DevIndex &ix, const uint2 *units, u
         a.ix = ix;
         a.units = units;
         a.n_units = n_units;
-        a.queue_head = &d->queue_head;
+        a.queue_heads = d->queue_heads;
+        a.n_parts = parts;
+        for (uint32_t p = 0; p < parts; p++)
+            a.part_end[p] = (uint32_t)((uint64_t)n_units * (p + 1) / parts);
         a.Av = Av;
         a.Bv = Bv;
         a.w = w;
