@@ -183,3 +183,27 @@ def test_queries_outside_the_grid_and_empty_requests(H, oracle, base):
         assert off[1] == 0 and off[2] == 0
         off, nb = S.result(S.submit(np.zeros((0, 6)), 0.03))
         assert list(off) == [0] and len(nb) == 0
+
+
+def test_large_groups_use_coarse_hit_bitmaps(H, oracle):
+    """Requests whose query groups see far more candidate chunks than their
+    hit bitmap has bits (4096 per query of the group): 1.2M 2-D points in a
+    0.03 square under cells of 0.01, so one query's stencil holds ~18,750
+    chunks (a bit covers 5 of them) and a group of 4 ~2 each.  The fill pass
+    must write exactly the rows the count pass sized."""
+    rng = np.random.default_rng(5)
+    pts = rng.random((1_200_000, 2)) * 0.03
+    eps = 0.0095
+    with H.Server(pts, [0.01], indexed_dims=2, slots=2, result_bytes=256 << 20, max_queries=16) as S:
+        for nq in (1, 4, 9):
+            print("request nq", nq, flush=True)
+            q = 0.0125 + rng.random((nq, 2)) * 0.005
+            t = S.submit(q, eps)
+            info = S.wait(t)
+            off, nb = S.result(t)
+            eoff, enb = oracle.range_join(pts, q, eps)
+            assert np.array_equal(off, eoff)
+            assert np.array_equal(sorted_rows(off, nb), enb)
+            assert info["pair_hash"] == oracle.csr_hash(eoff, enb)
+            assert info["evals"] >= nq * 900_000  # the stencils did hold the dense block
+
